@@ -1,0 +1,190 @@
+"""ctypes binding of the C ABI in ``include/biotfd.h`` (``libbiotfd.so``).
+
+The library is the HIP implementation; there is no CPU fallback. Loading fails
+loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import hostsetup
+from .formats import Job
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libbiotfd.so")
+
+FIELDS = ("U", "W", "QX", "QZ", "TXX", "TZZ", "TXZ", "P")
+EXPORTS = (
+    "bq_create", "bq_destroy", "bq_upload_model", "bq_build_pml", "bq_upload_pml", "bq_get_pml",
+    "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
+    "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
+    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times",
+)
+
+
+class Desc(C.Structure):
+    _fields_ = [("mx", C.c_int32), ("mz", C.c_int32), ("tpml", C.c_int32),
+                ("stress_imaging", C.c_int32), ("lin_x", C.c_int32), ("lin_z", C.c_int32),
+                ("ktbo", C.c_int32), ("mt2", C.c_int32), ("jmnum", C.c_int32),
+                ("reserved", C.c_int32), ("h", C.c_double), ("dt", C.c_double)]
+
+
+class BiotError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIBPATH):
+    """Load ``libbiotfd.so``; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BiotError(f"{path} is not built; run __graft_entry__.build() or "
+                        "python 2d-vqsp-fd-sim_amd/build.py")
+    lib = C.CDLL(path)
+    p = C.c_void_p
+    sig = {
+        "bq_create": ([C.POINTER(Desc), C.POINTER(p)], C.c_int),
+        "bq_destroy": ([p], None),
+        "bq_upload_model": ([p, p, p], C.c_int),
+        "bq_build_pml": ([p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+        "bq_upload_pml": ([p, p], C.c_int),
+        "bq_get_pml": ([p, C.c_int, p], C.c_int),
+        "bq_upload_bottom_nr": ([p, p, p], C.c_int),
+        "bq_set_source": ([p, p], C.c_int),
+        "bq_set_receivers": ([p, C.c_int, p, p, p, p], C.c_int),
+        "bq_run": ([p, C.c_int, C.c_int], C.c_int),
+        "bq_sync": ([p], C.c_int),
+        "bq_get_traces": ([p, p, C.c_int, C.c_int], C.c_int),
+        "bq_get_field": ([p, C.c_int, p], C.c_int),
+        "bq_set_field": ([p, C.c_int, p], C.c_int),
+        "bq_source_from_stf": ([p, p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, p],
+                               C.c_int),
+        "bq_last_error": ([p], C.c_char_p),
+        "bq_grid_shape": ([p, p, p], C.c_int),
+        "bq_enable_timing": ([p, C.c_int], C.c_int),
+        "bq_kernel_times": ([p, p], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def source_from_stf(job: Job) -> np.ndarray:
+    """OPEN_SR_FILE through the library's host helper: SOURTF_A as [MT2][2]."""
+    lib = load()
+    t, v = hostsetup.read_stf_reals(job, np.float32)
+    out = np.zeros((job.mt2, 2), dtype=np.float32)
+    rc = lib.bq_source_from_stf(_ptr(t), _ptr(v), len(t), job.mt2, np.float32(job.dt),
+                                np.float32(job.h), job.key_sour, np.float32(job.qr), _ptr(out))
+    if rc:
+        raise BiotError(f"bq_source_from_stf failed ({rc})")
+    return out
+
+
+class Simulation:
+    """One run on the GPU, driven like the reference's TIME_LOOP.
+
+    Set-up follows the reference driver's order (main.F90:60-84): model
+    (READ_MODEL, READ_MODEL_PAR), source (OPEN_SR_FILE), bottom coefficients
+    (PRE_NR_COEF1), PML (PPML); then :meth:`run` advances time steps.
+    """
+
+    def __init__(self, job: Job, abw=None, abh=None, pml_arrays=None, sourtf=None):
+        self.lib = load()
+        self.job = job
+        self.nz, self.nx = hostsetup.grid_shape(job)
+        d = Desc(job.mx, job.mzp, job.tpml, int(job.stress_imaging), job.lin_x, job.lin_z,
+                 job.ktbo, job.mt2, job.table.shape[0], 0, job.h, float(np.float32(job.dt)))
+        h = C.c_void_p()
+        rc = self.lib.bq_create(C.byref(d), C.byref(h))
+        if rc:
+            raise BiotError(f"bq_create failed ({rc})")
+        self.h = h
+        self.jm = hostsetup.jm_grid(job)
+        self.tab = np.ascontiguousarray(hostsetup.scaled_table(job, np.float32))
+        self._chk(self.lib.bq_upload_model(h, _ptr(self.jm), _ptr(self.tab)))
+        if pml_arrays is None:
+            self._chk(self.lib.bq_build_pml(h, job.r, job.tau, job.pow, job.sf, job.wc))
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in pml_arrays]
+            ptrs = (C.c_void_p * 24)(*[a.ctypes.data for a in arrs])
+            self._chk(self.lib.bq_upload_pml(h, ptrs))
+        abw = np.zeros(8, np.float32) if abw is None else np.asarray(abw, np.float32)
+        abh = np.zeros(8, np.float32) if abh is None else np.asarray(abh, np.float32)
+        self._chk(self.lib.bq_upload_bottom_nr(h, _ptr(abw), _ptr(abh)))
+        self.sourtf = source_from_stf(job) if sourtf is None else np.ascontiguousarray(sourtf, np.float32)
+        self._chk(self.lib.bq_set_source(h, _ptr(self.sourtf)))
+        rec = np.asarray(job.receivers, dtype=np.int32).reshape(-1, 4)
+        self.mr = rec.shape[0]
+        if self.mr:
+            cols = [np.ascontiguousarray(rec[:, k]) for k in range(4)]
+            self._chk(self.lib.bq_set_receivers(h, self.mr, *[_ptr(c) for c in cols]))
+        self.step = 0
+
+    def _chk(self, rc):
+        if rc:
+            msg = self.lib.bq_last_error(self.h)
+            raise BiotError(f"biotfd error {rc}: {msg.decode() if msg else ''}")
+
+    def run(self, nsteps: int, sync: bool = True):
+        self._chk(self.lib.bq_run(self.h, self.step + 1, self.step + nsteps))
+        self.step += nsteps
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._chk(self.lib.bq_sync(self.h))
+
+    def enable_timing(self, on: bool = True):
+        self._chk(self.lib.bq_enable_timing(self.h, int(on)))
+
+    def kernel_times(self) -> np.ndarray:
+        out = np.zeros(4, dtype=np.float64)
+        self._chk(self.lib.bq_kernel_times(self.h, _ptr(out)))
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.zeros((self.nz, self.nx), dtype=np.float32)
+        self._chk(self.lib.bq_get_field(self.h, FIELDS.index(name), _ptr(out)))
+        return out
+
+    def set_field(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._chk(self.lib.bq_set_field(self.h, FIELDS.index(name), _ptr(a)))
+
+    def traces(self, first: int = 1, n: int | None = None) -> np.ndarray:
+        n = self.step - first + 1 if n is None else n
+        out = np.zeros((n, self.mr, 6), dtype=np.float32)
+        self._chk(self.lib.bq_get_traces(self.h, _ptr(out), first, n))
+        return out
+
+    def pml(self, k: int) -> np.ndarray:
+        n = self.lib.bq_get_pml(self.h, k, None)
+        out = np.zeros(n, dtype=np.float32)
+        self._chk(min(0, self.lib.bq_get_pml(self.h, k, _ptr(out))))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,887 @@
+// bq_device.hip - CDNA4 (gfx950) kernels and C ABI of the Biot time loop.
+//
+// One time step of the reference (time_loop.F90:91-190, serial build, stress
+// imaging) becomes four launches on the handle's stream:
+//
+//   k_stress    rows 2..PMLZH   all strain and stress row kernels of one row
+//                               class fused per cell, plus the image rows 0/1
+//                               of the free surface (HPLANE_0/1_STRESS)
+//   k_velocity  rows 2..PMLZH-1 all INH_* updates of one row class fused per
+//                               cell; the previous filtration velocity is the
+//                               cell's own value before the update, so the
+//                               QXP/QZP copies of the reference are not stored
+//   k_edges     row 1 (HPLANE_1_VELOCITY) and row PMLZH
+//                               (HPLANE_PMLZH0_VELOCITY), which need the new
+//                               neighbouring rows, and the bottom history
+//   k_sample    receiver samples of the step
+//
+// Each cell is evaluated with the reference's operation order and no
+// floating-point contraction (the file is built with -ffp-contract=off), so
+// the fields are bit-identical to the source-order binary32 evaluation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/biotfd.h"
+#include "bq_internal.h"
+
+using bq::GridDesc;
+
+namespace {
+
+constexpr int kBX = 64;   // threads per block along x (one wavefront)
+constexpr int kBY = 4;    // rows per block
+constexpr int kEdgeBlock = 256;
+
+struct KParams {
+  int MX, MZ, T, PMLL, PMLZH, NX, P, OFF, LIN_X, LIN_Z, MR;
+  float DT, A, B, AH, BH, CH, BC;
+  float *U, *W, *QX, *QZ, *TXX, *TZZ, *TXZ, *PR;
+  const int *JM;
+  const float4 *stab;  // per material: {MU,SXX1,SXX2,SXX3},{SZZ1,SZZ2,SZZ3,P1},{P2,P3,-,-}
+  const float4 *vtab;  // per material: {eQX,R1QX,R2QX,fU},{R1U,R3U,eQZ,R1QZ},{R2QZ,fW,R1W,R3W}
+  const float2 *atab;  // per material: {AUXIL_W, AUXIL_QZ}
+  const float *x1[8], *x2[8], *z1[4], *z2[4];
+  float *PZXL, *PZXH, *PXXL, *PXXH, *OXXL, *OXXH, *PXZH, *PZZH, *OZZH;
+  float *RXXL, *RXXH, *SXXL, *SXXH, *GXXL, *GXXH, *HXXL, *HXXH;
+  float *RZXL, *RZXH, *GZXL, *GZXH;
+  float *RXZH, *GXZH, *RZZH, *SZZH, *GZZH, *HZZH;
+  float *SZc, *SZp;     // bottom history: [4 fields][NX][3] current / previous
+  const float2 *dsrc;   // per step: ((S-Sprev)/DT, (F-Fprev)/DT)
+  const int4 *rec;      // receivers: irecx, lrecx, irecz, lrecz
+  float *traces;        // [MT2][MR][6]
+  float abw[8], abh[8];
+};
+
+#define FA(a, i, l) (a)[(size_t)(l) * (size_t)k.P + (size_t)((i) + k.OFF)]
+#define XL1(t, j, l) (k.x1[t][(l) * (T + 1) + (j)])
+#define XL2(t, j, l) (k.x2[t][(l) * (T + 2) + (j)])
+#define ZH1(t, j, ix) (k.z1[t][(ix) * (T + 1) + (j)])
+#define ZH2(t, j, ix) (k.z2[t][(ix) * (T + 2) + (j)])
+
+// ------------------------------------------------------------------ stress
+// One cell (i,l) of the stress half-step, l in [2, PMLZH]:
+//   l in [2, MZ'-2]  HPLANE_IC_STRESS / HPLANE_LIN_STRESS (hplane_ic_stress.F90:48-117)
+//   l = MZ'-1        HPLANE_MZ1_STRESS (hplane_mz1_stress.F90:50-130)
+//   l in [MZ', PMLZH-1]  HPLANE_PML_STRESS, LL = l-MZ'+1 (hplane_pml_stress.F90:53-134)
+//   l = PMLZH        HPLANE_PMLZH0_STRESS, TXZ only (hplane_pmlzh0_stress.F90:43-68)
+// and the stress-imaging rows (hplane_0_stress.F90:47-51, hplane_1_stress.F90:44-49).
+__global__ __launch_bounds__(kBX * kBY) void k_stress(KParams k, int itile) {
+  const int ix = blockIdx.x * kBX + threadIdx.x;
+  const int l = 2 + blockIdx.y * kBY + threadIdx.y;
+  if (ix >= k.NX || l > k.PMLZH) return;
+  const int T = k.T, MX = k.MX, MZ = k.MZ, PMLZH = k.PMLZH, NX = k.NX;
+  const int i = ix + k.PMLL;
+  const bool zh0 = (l == PMLZH);
+  const bool pmlrow = (l >= MZ && l < PMLZH);
+  const bool zpml = zh0 || pmlrow;
+  const int LL = zh0 ? T + 1 : (pmlrow ? l - MZ + 1 : 0);
+  const float C = k.CH, DT = k.DT;
+  const int jm = k.JM[(size_t)l * k.P + ix] - 1;
+  const float4 s0 = k.stab[3 * jm];
+
+  // ---- shear strain and TXZ: STRAIN_UW_XZ (strain_uw_xz.F90:61-103) or
+  //      STRAIN_UW_XZ_2ND_PML (strain_uw_xz_2nd_pml.F90:69-121), STRESS_XZ
+  float txz = FA(k.TXZ, i, l);
+  if (i >= 2 - T && i <= MX + T) {
+    float exz;
+    if (!zpml) {
+      if (i <= 2) {
+        const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
+        float &pm = k.PZXL[(size_t)l * (T + 2) + ix];
+        const int j = 3 - i;
+        exz = C * (FA(k.U, i, l) - FA(k.U, i, l - 1)) + XL2(0, j, l) * (XL2(3, j, l) * dzxa + pm);
+        pm = XL2(1, j, l) * pm + XL2(2, j, l) * dzxa;
+      } else if (i <= MX - 1) {
+        exz = k.AH * (FA(k.U, i, l + 1) - FA(k.U, i, l - 2) + FA(k.W, i + 1, l) - FA(k.W, i - 2, l)) +
+              k.BH * (FA(k.U, i, l) - FA(k.U, i, l - 1) + FA(k.W, i, l) - FA(k.W, i - 1, l));
+      } else {
+        const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
+        float &pm = k.PZXH[(size_t)l * (T + 1) + (i - MX)];
+        const int j = i - (MX - 1);
+        exz = C * (FA(k.U, i, l) - FA(k.U, i, l - 1)) + XL2(4, j, l) * (XL2(7, j, l) * dzxa + pm);
+        pm = XL2(5, j, l) * pm + XL2(6, j, l) * dzxa;
+      }
+    } else {
+      const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
+      const float dxza = C * (FA(k.U, i, l) - FA(k.U, i, l - 1));
+      float &pz = k.PXZH[(size_t)LL * NX + ix];
+      const float zp = ZH2(0, LL, ix) * (ZH2(3, LL, ix) * dxza + pz);
+      if (i <= 2) {
+        float &pm = k.PZXL[(size_t)l * (T + 2) + ix];
+        const int j = 3 - i;
+        exz = zp + XL2(0, j, l) * (XL2(3, j, l) * dzxa + pm);
+        pm = XL2(1, j, l) * pm + XL2(2, j, l) * dzxa;
+      } else if (i <= MX - 1) {
+        exz = zp + dzxa;
+      } else {
+        float &pm = k.PZXH[(size_t)l * (T + 1) + (i - MX)];
+        const int j = i - (MX - 1);
+        exz = zp + XL2(4, j, l) * (XL2(7, j, l) * dzxa + pm);
+        pm = XL2(5, j, l) * pm + XL2(6, j, l) * dzxa;
+      }
+      pz = ZH2(1, LL, ix) * pz + ZH2(2, LL, ix) * dxza;
+    }
+    txz = txz + DT * (s0.x * exz);
+  }
+  // TXZ(:,2) = 0 after the images are taken (hplane_1_stress.F90:48)
+  FA(k.TXZ, i, l) = (l == 2) ? 0.0f : txz;
+  if (l == 3) FA(k.TXZ, i, 1) = -txz;
+  if (l == 4) FA(k.TXZ, i, 0) = -txz;
+  if (zh0) return;
+
+  // ---- normal strains and TXX, TZZ, PRES: STRAIN_UW_XX / STRAIN_Q_XX
+  //      (strain_uw_xx.F90:60-107), STRAIN_UW_ZZ / STRAIN_Q_ZZ (strain_uw_zz.F90:49-60)
+  //      or their _2ND_PML forms (strain_uw_zz_2nd_pml.F90:55-65), STRESS_NN /
+  //      STRESS_LIN_NN (stress_nn.F90:49-61, stress_lin_nn.F90:49-70)
+  float tzz = FA(k.TZZ, i, l), pr = FA(k.PR, i, l);
+  if (i >= 2 - T && i <= MX - 1 + T) {
+    float exx, qxx, ezz, qzz;
+    if (i <= 1) {
+      const int j = 2 - i;
+      const size_t mo = (size_t)l * (T + 1) + ix;
+      const float a = XL1(0, j, l), b = XL1(1, j, l), c = XL1(2, j, l), d = XL1(3, j, l);
+      const float du = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
+      const float dq = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      float pu = k.PXXL[mo], pq = k.OXXL[mo];
+      exx = a * (d * du + pu);
+      qxx = a * (d * dq + pq);
+      k.PXXL[mo] = b * pu + c * du;
+      k.OXXL[mo] = b * pq + c * dq;
+    } else if (i == 2 || i == MX - 1) {
+      exx = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
+      qxx = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+    } else if (i <= MX - 2) {
+      exx = k.AH * (FA(k.U, i + 2, l) - FA(k.U, i - 1, l)) + k.BH * (FA(k.U, i + 1, l) - FA(k.U, i, l));
+      qxx = k.AH * (FA(k.QX, i + 2, l) - FA(k.QX, i - 1, l)) + k.BH * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+    } else {
+      const int j = i - (MX - 1);
+      const size_t mo = (size_t)l * (T + 1) + (i - MX);
+      const float a = XL1(4, j, l), b = XL1(5, j, l), c = XL1(6, j, l), d = XL1(7, j, l);
+      const float du = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
+      const float dq = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      float pu = k.PXXH[mo], pq = k.OXXH[mo];
+      exx = a * (d * du + pu);
+      qxx = a * (d * dq + pq);
+      k.PXXH[mo] = b * pu + c * du;
+      k.OXXH[mo] = b * pq + c * dq;
+    }
+    if (!pmlrow && l != MZ - 1) {
+      ezz = (C * (FA(k.W, i, l + 1) - FA(k.W, i, l)));
+      qzz = (C * (FA(k.QZ, i, l + 1) - FA(k.QZ, i, l)));
+      if (i >= 2 && i <= MX - 1) {
+        ezz = k.AH * (FA(k.W, i, l + 2) - FA(k.W, i, l - 1)) + k.BC * ezz;
+        qzz = k.AH * (FA(k.QZ, i, l + 2) - FA(k.QZ, i, l - 1)) + k.BC * qzz;
+      }
+    } else {
+      const float a = ZH1(0, LL, ix), b = ZH1(1, LL, ix), c = ZH1(2, LL, ix), d = ZH1(3, LL, ix);
+      const float dw = C * (FA(k.W, i, l + 1) - FA(k.W, i, l));
+      const float dq = C * (FA(k.QZ, i, l + 1) - FA(k.QZ, i, l));
+      const size_t mo = (size_t)LL * NX + ix;
+      float pw = k.PZZH[mo], pq = k.OZZH[mo];
+      ezz = a * (d * dw + pw);
+      qzz = a * (d * dq + pq);
+      k.PZZH[mo] = b * pw + c * dw;
+      k.OZZH[mo] = b * pq + c * dq;
+    }
+    const float4 s1 = k.stab[3 * jm + 1];
+    const float4 s2 = k.stab[3 * jm + 2];
+    const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
+    const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
+    const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
+    float txx = FA(k.TXX, i, l);
+    if (l == k.LIN_Z && i == k.LIN_X) {
+      const float2 ds = k.dsrc[itile - 1];
+      txx = txx + DT * (rxx - ds.x);
+      tzz = tzz + DT * (rzz - ds.x);
+      pr = pr + DT * (rpp + ds.y);
+    } else {
+      txx = txx + DT * rxx;
+      tzz = tzz + DT * rzz;
+      pr = pr + DT * rpp;
+    }
+    FA(k.TXX, i, l) = txx;
+    FA(k.TZZ, i, l) = tzz;
+    FA(k.PR, i, l) = pr;
+  }
+  if (l == 2) { FA(k.TZZ, i, 1) = -tzz; FA(k.PR, i, 1) = -pr; }
+  if (l == 3) { FA(k.TZZ, i, 0) = -tzz; FA(k.PR, i, 0) = -pr; }
+}
+
+// ---------------------------------------------------------------- velocity
+// One cell (i,l) of the velocity half-step, l in [2, PMLZH-1]:
+//   l in [2, MZ'-2]        HPLANE_IC_VELOCITY (hplane_ic_velocity.F90:59-202):
+//                          INH_QX_INT, INH_V_UV_INT (inh_qx_int.F90:40-54,
+//                          inh_v_uv_int.F90:41-56) inside, INH_QX_2ND /
+//                          INH_V_U_2ND (inh_qx_2nd.F90:54-103, inh_v_u_2nd.F90:56-104)
+//                          in the side PML; likewise QZ/W
+//   l in [MZ'-1, PMLZH-1]  HPLANE_PML_VELOCITY, LL = l-MZ'+1 (hplane_pml_velocity.F90:64-180)
+__global__ __launch_bounds__(kBX * kBY) void k_velocity(KParams k) {
+  const int ix = blockIdx.x * kBX + threadIdx.x;
+  const int l = 2 + blockIdx.y * kBY + threadIdx.y;
+  if (ix >= k.NX || l > k.PMLZH - 1) return;
+  const int T = k.T, MX = k.MX, NX = k.NX;
+  const int i = ix + k.PMLL;
+  const bool pmlrow = (l >= k.MZ - 1);
+  const int LL = l - k.MZ + 1;
+  const float A = k.A, B = k.B;
+  const int jm = k.JM[(size_t)l * k.P + ix] - 1;
+  const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
+  // v0 = {eQX, R1QX, R2QX, fU}, v1 = {R1U, R3U, eQZ, R1QZ}, v2 = {R2QZ, fW, R1W, R3W}
+
+  // ---- QX and U, x in [3-T, MX-1+T]
+  if (i >= 3 - T && i <= MX - 1 + T) {
+    const float q = FA(k.QX, i, l);
+    const float u = FA(k.U, i, l);
+    float qn, un;
+    const bool inner = (i >= 4 && i <= MX - 2);
+    if (!pmlrow) {
+      if (inner) {
+        const float sx = A * (FA(k.TXX, i + 1, l) - FA(k.TXX, i - 2, l) + FA(k.TXZ, i, l + 2) - FA(k.TXZ, i, l - 1)) +
+                         B * (FA(k.TXX, i, l) - FA(k.TXX, i - 1, l) + FA(k.TXZ, i, l + 1) - FA(k.TXZ, i, l));
+        const float px = A * (FA(k.PR, i + 1, l) - FA(k.PR, i - 2, l)) + B * (FA(k.PR, i, l) - FA(k.PR, i - 1, l));
+        qn = v0.x * q - v0.y * sx - v0.z * px;
+        un = u + v0.w * q + v1.x * sx + v1.y * px;
+      } else {
+        const bool right = i > 3;
+        const int j = right ? i - (MX - 1) : 3 - i;
+        const int t0 = right ? 4 : 0;
+        const float oa = XL2(t0, j, l), ob = XL2(t0 + 1, j, l), oc = XL2(t0 + 2, j, l), od = XL2(t0 + 3, j, l);
+        const size_t mo = right ? (size_t)l * (T + 2) + (i - (MX - 1)) : (size_t)l * (T + 3) + ix;
+        float *G = right ? k.GXXH : k.GXXL, *H = right ? k.HXXH : k.HXXL;
+        float *R = right ? k.RXXH : k.RXXL, *S = right ? k.SXXH : k.SXXL;
+        const float g = G[mo], hh = H[mo], r = R[mo], s = S[mo];
+        const float dxx = FA(k.TXX, i, l) - FA(k.TXX, i - 1, l);
+        const float pxx = FA(k.PR, i, l) - FA(k.PR, i - 1, l);
+        const float tz1 = FA(k.TXZ, i, l + 1), tz0 = FA(k.TXZ, i, l);
+        qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1 - tz0) - v0.z * (oa * (od * pxx + hh));
+        un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1 - tz0) + v1.y * (oa * (od * pxx + s));
+        G[mo] = ob * g + oc * dxx;
+        H[mo] = ob * hh + oc * pxx;
+        R[mo] = ob * r + oc * dxx;
+        S[mo] = ob * s + oc * pxx;
+      }
+    } else {
+      // INH_QX_INT_2ND_PML (inh_qx_int_2nd_pml.F90:60-143), INH_V_U_INT_2ND_PML
+      // (inh_v_u_int_2nd_pml.F90:62-143)
+      const float dxx = FA(k.TXX, i, l) - FA(k.TXX, i - 1, l);
+      const float dxz = FA(k.TXZ, i, l + 1) - FA(k.TXZ, i, l);
+      const float pxx = FA(k.PR, i, l) - FA(k.PR, i - 1, l);
+      const float za = ZH1(0, LL, ix), zb = ZH1(1, LL, ix), zc = ZH1(2, LL, ix), zd = ZH1(3, LL, ix);
+      const size_t zo = (size_t)LL * NX + ix;
+      const float gz = k.GXZH[zo], rz = k.RXZH[zo];
+      if (inner) {
+        qn = v0.x * q - v0.y * (za * (zd * dxz + gz) + dxx) - v0.z * pxx;
+        un = u + v0.w * q + v1.x * (dxx + za * (zd * dxz + rz)) + v1.y * pxx;
+      } else {
+        const bool right = i > 3;
+        const int j = right ? i - (MX - 1) : 3 - i;
+        const int t0 = right ? 4 : 0;
+        const float oa = XL2(t0, j, l), ob = XL2(t0 + 1, j, l), oc = XL2(t0 + 2, j, l), od = XL2(t0 + 3, j, l);
+        const size_t mo = right ? (size_t)l * (T + 2) + (i - (MX - 1)) : (size_t)l * (T + 3) + ix;
+        float *G = right ? k.GXXH : k.GXXL, *H = right ? k.HXXH : k.HXXL;
+        float *R = right ? k.RXXH : k.RXXL, *S = right ? k.SXXH : k.SXXL;
+        const float g = G[mo], hh = H[mo], r = R[mo], s = S[mo];
+        qn = v0.x * q - v0.y * (oa * (od * dxx + g)) - v0.y * (za * (zd * dxz + gz)) - v0.z * (oa * (od * pxx + hh));
+        un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + za * (zd * dxz + rz)) + v1.y * (oa * (od * pxx + s));
+        G[mo] = ob * g + oc * dxx;
+        H[mo] = ob * hh + oc * pxx;
+        R[mo] = ob * r + oc * dxx;
+        S[mo] = ob * s + oc * pxx;
+      }
+      k.GXZH[zo] = zb * gz + zc * dxz;
+      k.RXZH[zo] = zb * rz + zc * dxz;
+    }
+    FA(k.QX, i, l) = qn;
+    FA(k.U, i, l) = un;
+  }
+
+  // ---- QZ and W, x in [2-T, MX-1+T]
+  if (i >= 2 - T && i <= MX - 1 + T) {
+    const float q = FA(k.QZ, i, l);
+    const float w = FA(k.W, i, l);
+    float qn, wn;
+    const bool inner = (i >= 3 && i <= MX - 2);
+    if (!pmlrow) {
+      if (inner) {
+        // INH_QZ_INT (inh_qz_int.F90:41-55), INH_V_W_INT (inh_v_w_int.F90:43-58)
+        const float sz = A * (FA(k.TZZ, i, l + 1) - FA(k.TZZ, i, l - 2) + FA(k.TXZ, i + 2, l) - FA(k.TXZ, i - 1, l)) +
+                         B * (FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1) + FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l));
+        const float pz = A * (FA(k.PR, i, l + 1) - FA(k.PR, i, l - 2)) + B * (FA(k.PR, i, l) - FA(k.PR, i, l - 1));
+        qn = v1.z * q - v1.w * sz - v2.x * pz;
+        wn = w + v2.y * q + v2.z * sz + v2.w * pz;
+      } else {
+        // INH_QZ_2ND (inh_qz_2nd.F90:49-88), INH_V_W_2ND (inh_v_w_2nd.F90:52-91)
+        const bool right = i > 2;
+        const int j = right ? i - (MX - 1) : 2 - i;
+        const int t0 = right ? 4 : 0;
+        const float oa = XL1(t0, j, l), ob = XL1(t0 + 1, j, l), oc = XL1(t0 + 2, j, l), od = XL1(t0 + 3, j, l);
+        const size_t mo = right ? (size_t)l * (T + 2) + (i - (MX - 1)) : (size_t)l * (T + 2) + ix;
+        float *G = right ? k.GZXH : k.GZXL, *R = right ? k.RZXH : k.RZXL;
+        const float g = G[mo], r = R[mo];
+        const float dxz = FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l);
+        const float dzz = FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1);
+        const float pzz = FA(k.PR, i, l) - FA(k.PR, i, l - 1);
+        qn = v1.z * q - v1.w * (dzz + oa * (od * dxz + g)) - v2.x * pzz;
+        wn = w + v2.y * q + v2.z * (dzz + oa * (od * dxz + r)) + v2.w * pzz;
+        G[mo] = ob * g + oc * dxz;
+        R[mo] = ob * r + oc * dxz;
+      }
+    } else {
+      // INH_QZ_INT_2ND_PML (inh_qz_int_2nd_pml.F90:57-144), INH_V_W_INT_2ND_PML
+      // (inh_v_w_int_2nd_pml.F90:61-146)
+      const float dxz = FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l);
+      const float dzz = FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1);
+      const float pzz = FA(k.PR, i, l) - FA(k.PR, i, l - 1);
+      const float za = ZH2(0, LL, ix), zb = ZH2(1, LL, ix), zc = ZH2(2, LL, ix), zd = ZH2(3, LL, ix);
+      const size_t zo = (size_t)LL * NX + ix;
+      const float gz = k.GZZH[zo], hz = k.HZZH[zo], rz = k.RZZH[zo], sz = k.SZZH[zo];
+      if (inner) {
+        qn = v1.z * q - v1.w * (za * (zd * dzz + gz) + dxz) - v2.x * (za * (zd * pzz + hz));
+        wn = w + v2.y * q + v2.z * (za * (zd * dzz + rz) + dxz) + v2.w * (za * (zd * pzz + sz));
+      } else {
+        const bool right = i > 2;
+        const int j = right ? i - (MX - 1) : 2 - i;
+        const int t0 = right ? 4 : 0;
+        const float oa = XL1(t0, j, l), ob = XL1(t0 + 1, j, l), oc = XL1(t0 + 2, j, l), od = XL1(t0 + 3, j, l);
+        const size_t mo = right ? (size_t)l * (T + 2) + (i - (MX - 1)) : (size_t)l * (T + 2) + ix;
+        float *G = right ? k.GZXH : k.GZXL, *R = right ? k.RZXH : k.RZXL;
+        const float g = G[mo], r = R[mo];
+        qn = v1.z * q - v1.w * (za * (zd * dzz + gz)) - v1.w * (oa * (od * dxz + g)) - v2.x * (za * (zd * pzz + hz));
+        wn = w + v2.y * q + v2.z * (za * (zd * dzz + rz) + oa * (od * dxz + r)) + v2.w * (za * (zd * pzz + sz));
+        G[mo] = ob * g + oc * dxz;
+        R[mo] = ob * r + oc * dxz;
+      }
+      k.GZZH[zo] = zb * gz + zc * dzz;
+      k.HZZH[zo] = zb * hz + zc * pzz;
+      k.RZZH[zo] = zb * rz + zc * dzz;
+      k.SZZH[zo] = zb * sz + zc * pzz;
+    }
+    FA(k.QZ, i, l) = qn;
+    FA(k.W, i, l) = wn;
+  }
+}
+
+// ------------------------------------------------------------------- edges
+// Row 1, stress imaging (HPLANE_1_VELOCITY, hplane_1_velocity.F90:61-76):
+// INH_V_QX_INT_1_SI (inh_v_qx_int_1_si.F90:26-31) for all x first, then
+// INH_W_QZ_INT_1_SI (inh_w_qz_int_1_si.F90:30-44), which reads the new row-1
+// U and QX at i+1: each block recomputes the one halo column it needs.
+// Row PMLZH (HPLANE_PMLZH0_VELOCITY, hplane_pmlzh0_velocity.F90:34-63) from
+// the new rows PMLZH-1, PMLZH-2 and the two stored levels, then the history
+// is advanced, as the store of time_loop.F90:133-155 sees it next step.
+__global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
+  __shared__ float su[kEdgeBlock + 1], sq[kEdgeBlock + 1];
+  const int T = k.T, MX = k.MX, NX = k.NX;
+  const int ix0 = blockIdx.x * kEdgeBlock;
+  const int ix = ix0 + threadIdx.x;
+  // phase 1: U(i,1), QX(i,1) for the block's columns and one halo column
+  for (int t = threadIdx.x; t <= kEdgeBlock; t += kEdgeBlock) {
+    const int jx = ix0 + t;
+    const int i = jx + k.PMLL;
+    if (jx < NX && i >= 3 - T && i <= MX - 1 + T) {
+      const float u1 = FA(k.U, i, 2) + FA(k.W, i, 2) - FA(k.W, i - 1, 2);
+      const float q1 = FA(k.QX, i, 2) + FA(k.QZ, i, 2) - FA(k.QZ, i - 1, 2);
+      su[t] = u1;
+      sq[t] = q1;
+    }
+  }
+  __syncthreads();
+  if (ix < NX) {
+    const int i = ix + k.PMLL;
+    if (i >= 3 - T && i <= MX - 1 + T) {
+      FA(k.U, i, 1) = su[threadIdx.x];
+      FA(k.QX, i, 1) = sq[threadIdx.x];
+    }
+    // phase 2
+    if (i >= 3 - T && i <= MX - 2 + T) {
+      const int jm = k.JM[ix] - 1;  // JM(I,0)
+      const float2 aux = k.atab[jm];
+      const float u0 = su[threadIdx.x], u1 = su[threadIdx.x + 1];
+      const float q0 = sq[threadIdx.x], q1 = sq[threadIdx.x + 1];
+      const float du = u1 - u0 + FA(k.U, i + 1, 2) - FA(k.U, i, 2);
+      FA(k.W, i, 1) = FA(k.W, i, 3) + aux.x * du;
+      FA(k.QZ, i, 1) = FA(k.QZ, i, 3) + aux.y * du + (q1 - q0 + FA(k.QX, i + 1, 2) - FA(k.QX, i, 2));
+    }
+    // bottom row
+    const int lst = k.PMLZH;
+    if (i >= 1 - T && i <= MX + T) {
+      float *F[4] = {k.U, k.W, k.QX, k.QZ};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const float *c = (f == 1 || f == 3) ? k.abw : k.abh;
+        float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+        float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+        const float f1 = FA(F[f], i, lst - 1), f2 = FA(F[f], i, lst - 2);
+        const float n = c[0] * f1 + c[1] * f2 + c[2] * sc[0] + c[3] * sc[1] + c[4] * sc[2] +
+                        c[5] * sp[0] + c[6] * sp[1] + c[7] * sp[2];
+        FA(F[f], i, lst) = n;
+        sp[0] = sc[0]; sp[1] = sc[1]; sp[2] = sc[2];
+        sc[0] = n; sc[1] = f1; sc[2] = f2;
+      }
+    } else {
+      // columns outside the bottom update keep their values; the history still
+      // records them (time_loop.F90:140-153 copies whole rows)
+      float *F[4] = {k.U, k.W, k.QX, k.QZ};
+      for (int f = 0; f < 4; ++f) {
+        float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+        float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+        sp[0] = sc[0]; sp[1] = sc[1]; sp[2] = sc[2];
+        sc[0] = FA(F[f], i, lst); sc[1] = FA(F[f], i, lst - 1); sc[2] = FA(F[f], i, lst - 2);
+      }
+    }
+  }
+}
+
+// Receiver samples (store_seismograms.F90:58-83, store_seismograms_pressure.F90:50-56)
+__global__ void k_sample(KParams k, int itile) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k.MR) return;
+  const int4 r = k.rec[j];
+  const int nzl = k.PMLZH, lo = k.PMLL, hi = k.PMLL + k.NX - 1;
+  const bool okx = r.x >= lo && r.x <= hi && r.y >= 0 && r.y <= nzl;
+  const bool okz = r.z >= lo && r.z <= hi && r.w >= 0 && r.w <= nzl;
+  const bool okp = r.z >= lo && r.z <= hi && r.y >= 0 && r.y <= nzl;
+  float *o = k.traces + ((size_t)(itile - 1) * k.MR + j) * 6;
+  o[0] = okx ? FA(k.U, r.x, r.y) : 0.0f;
+  o[1] = okz ? -FA(k.W, r.z, r.w) : 0.0f;
+  o[2] = okx ? FA(k.QX, r.x, r.y) : 0.0f;
+  o[3] = okz ? -FA(k.QZ, r.z, r.w) : 0.0f;
+  o[4] = okp ? FA(k.PR, r.z, r.y) : 0.0f;
+  o[5] = okp ? -FA(k.TXX, r.z, r.y) : 0.0f;
+}
+
+#undef FA
+
+}  // namespace
+
+// ===================================================================== host
+
+struct bq_handle {
+  GridDesc g{};
+  int P = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  KParams kp{};
+  std::vector<void *> allocs;
+  float *fields = nullptr;
+  std::vector<int32_t> jm_host;
+  std::vector<float> tab_host;
+  bool model_ok = false, pml_ok = false, src_ok = false;
+  int next_step = 1;
+  int mr = 0;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double ktime[4] = {0, 0, 0, 0};
+  int timed_steps = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(bq_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                             \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(h, -100, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <class Tp>
+int dalloc(bq_handle *h, Tp **p, size_t n) {
+  void *q = nullptr;
+  size_t bytes = n ? n * sizeof(Tp) : 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(h, -101, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemsetAsync(q, 0, bytes, h->stream);
+  if (e != hipSuccess) return fail(h, -101, std::string("hipMemset: ") + hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *p = (Tp *)q;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bq_create(const bq_desc *d, bq_handle **out) {
+  if (!d || !out) return -1;
+  *out = nullptr;
+  if (d->mx < 8 || d->mz < 6 || d->tpml < 1 || d->tpml > 256 || d->jmnum < 1 || d->mt2 < 1)
+    return -2;
+  if (!d->stress_imaging) return -3;
+  if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
+      d->lin_x > d->mx - 1 + d->tpml)
+    return -4;
+  if (d->reserved != 0) return -1;
+  const long long nx = (long long)d->mx + 2LL * d->tpml;
+  const long long nz = (long long)d->mz + d->tpml + 1;
+  const long long pitch = (nx + 63) / 64 * 64;
+  if (pitch * nz >= (1LL << 31)) return -2;
+  bq_handle *h = new bq_handle();
+  GridDesc &g = h->g;
+  g.MX = d->mx; g.MZ = d->mz; g.T = d->tpml; g.PMLL = 1 - g.T; g.PMLXH = g.MX + g.T;
+  g.PMLZH = g.MZ + g.T; g.NX = (int)nx; g.NZ = (int)nz; g.LIN_X = d->lin_x; g.LIN_Z = d->lin_z;
+  g.KTBO = d->ktbo; g.MT2 = d->mt2; g.JMNUM = d->jmnum;
+  g.H = (float)d->h; g.DT = (float)d->dt;
+  h->P = (int)pitch;
+  (void)hipGetDevice(&h->device);
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete h; return -100; }
+  KParams &k = h->kp;
+  k.MX = g.MX; k.MZ = g.MZ; k.T = g.T; k.PMLL = g.PMLL; k.PMLZH = g.PMLZH; k.NX = g.NX;
+  k.P = h->P; k.OFF = -g.PMLL; k.LIN_X = g.LIN_X; k.LIN_Z = g.LIN_Z; k.MR = 0;
+  // MODULE AUXIL constants (mod_auxil.F90:19-20) and AUX1 (aux1.F90:44-46)
+  k.DT = g.DT;
+  k.A = -1.0f / 24.0f;
+  k.B = 9.0f / 8.0f;
+  k.AH = k.A / g.H;
+  k.BH = k.B / g.H;
+  k.CH = 1.0f / g.H;
+  k.BC = k.BH / k.CH;  // B/C of strain_uw_zz.F90:43
+  const int T = g.T, NX = g.NX, NZ = g.NZ;
+  int rc = 0;
+  const size_t fsz = (size_t)h->P * NZ;
+  rc |= dalloc(h, &h->fields, 8 * fsz);
+  if (rc) { bq_destroy(h); return rc; }
+  float **fp[8] = {&k.U, &k.W, &k.QX, &k.QZ, &k.TXX, &k.TZZ, &k.TXZ, &k.PR};
+  for (int f = 0; f < 8; ++f) *fp[f] = h->fields + f * fsz;
+  int *jmd = nullptr;
+  rc |= dalloc(h, &jmd, fsz);
+  k.JM = jmd;
+  float4 *st = nullptr, *vt = nullptr;
+  float2 *at = nullptr;
+  rc |= dalloc(h, &st, 3 * (size_t)g.JMNUM);
+  rc |= dalloc(h, &vt, 3 * (size_t)g.JMNUM);
+  rc |= dalloc(h, &at, (size_t)g.JMNUM);
+  k.stab = st; k.vtab = vt; k.atab = at;
+  for (int t = 0; t < 8; ++t) {
+    float *a = nullptr, *b = nullptr;
+    rc |= dalloc(h, &a, (size_t)(T + 1) * NZ);
+    rc |= dalloc(h, &b, (size_t)(T + 2) * NZ);
+    k.x1[t] = a; k.x2[t] = b;
+  }
+  for (int t = 0; t < 4; ++t) {
+    float *a = nullptr, *b = nullptr;
+    rc |= dalloc(h, &a, (size_t)(T + 1) * NX);
+    rc |= dalloc(h, &b, (size_t)(T + 2) * NX);
+    k.z1[t] = a; k.z2[t] = b;
+  }
+  // PML memories, shapes of ppml.F90:88-112
+  rc |= dalloc(h, &k.PZXL, (size_t)(T + 2) * NZ);
+  rc |= dalloc(h, &k.PZXH, (size_t)(T + 1) * NZ);
+  rc |= dalloc(h, &k.PXXL, (size_t)(T + 1) * NZ);
+  rc |= dalloc(h, &k.PXXH, (size_t)(T + 1) * NZ);
+  rc |= dalloc(h, &k.OXXL, (size_t)(T + 1) * NZ);
+  rc |= dalloc(h, &k.OXXH, (size_t)(T + 1) * NZ);
+  rc |= dalloc(h, &k.PXZH, (size_t)(T + 2) * NX);
+  rc |= dalloc(h, &k.PZZH, (size_t)(T + 1) * NX);
+  rc |= dalloc(h, &k.OZZH, (size_t)(T + 1) * NX);
+  float **l3[4] = {&k.RXXL, &k.SXXL, &k.GXXL, &k.HXXL};
+  for (auto p : l3) rc |= dalloc(h, p, (size_t)(T + 3) * NZ);
+  float **r2[4] = {&k.RXXH, &k.SXXH, &k.GXXH, &k.HXXH};
+  for (auto p : r2) rc |= dalloc(h, p, (size_t)(T + 2) * NZ);
+  float **z2m[4] = {&k.RZXL, &k.GZXL, &k.RZXH, &k.GZXH};
+  for (auto p : z2m) rc |= dalloc(h, p, (size_t)(T + 2) * NZ);
+  float **zb[6] = {&k.RXZH, &k.GXZH, &k.RZZH, &k.SZZH, &k.GZZH, &k.HZZH};
+  for (auto p : zb) rc |= dalloc(h, p, (size_t)(T + 1) * NX);
+  rc |= dalloc(h, &k.SZc, (size_t)4 * NX * 3);
+  rc |= dalloc(h, &k.SZp, (size_t)4 * NX * 3);
+  float2 *ds = nullptr;
+  rc |= dalloc(h, &ds, (size_t)g.MT2);
+  k.dsrc = ds;
+  if (rc) { int r = rc; bq_destroy(h); return r < 0 ? r : -101; }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) { bq_destroy(h); return -100; }
+  *out = h;
+  return 0;
+}
+
+void bq_destroy(bq_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) (void)hipFree(p);
+  for (auto e : h->ev) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char *bq_last_error(bq_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz) {
+  if (!h) return -1;
+  if (nx) *nx = h->g.NX;
+  if (nz) *nz = h->g.NZ;
+  return 0;
+}
+
+int bq_upload_model(bq_handle *h, const int32_t *jm, const float *tab) {
+  if (!h || !jm || !tab) return fail(h, -1, "null argument");
+  const GridDesc &g = h->g;
+  const size_t ncell = (size_t)g.NX * g.NZ;
+  for (size_t c = 0; c < ncell; ++c)
+    if (jm[c] < 1 || jm[c] > g.JMNUM) return fail(h, -5, "material index out of range");
+  h->jm_host.assign(jm, jm + ncell);
+  h->tab_host.assign(tab, tab + (size_t)g.JMNUM * 24);
+  // jm padded to the device pitch
+  std::vector<int32_t> jp((size_t)h->P * g.NZ, 1);
+  for (int l = 0; l < g.NZ; ++l)
+    std::memcpy(&jp[(size_t)l * h->P], jm + (size_t)l * g.NX, sizeof(int32_t) * g.NX);
+  std::vector<float> st(12 * (size_t)g.JMNUM, 0.0f), vt(12 * (size_t)g.JMNUM, 0.0f);
+  std::vector<float> at(2 * (size_t)g.JMNUM, 0.0f);
+  using namespace bq;
+  for (int m = 0; m < g.JMNUM; ++m) {
+    const float *t = tab + (size_t)m * 24;
+    float *s = &st[12 * (size_t)m];
+    s[0] = t[T_MU]; s[1] = t[T_SXX1]; s[2] = t[T_SXX2]; s[3] = t[T_SXX3];
+    s[4] = t[T_SZZ1]; s[5] = t[T_SZZ2]; s[6] = t[T_SZZ3]; s[7] = t[T_P1];
+    s[8] = t[T_P2]; s[9] = t[T_P3];
+    // friction factors in the reference's binary32 order (inh_qx_int.F90:44,
+    // inh_v_uv_int.F90:46, inh_qz_int.F90:45, inh_v_w_int.F90:48)
+    const float eqx = expf(-t[T_REL3_QX]);
+    const float eqz = expf(-t[T_REL3_QZ]);
+    const float fu = (t[T_REL2_U] / t[T_REL3_QX]) * (1.0f - expf(-t[T_REL3_QX]));
+    const float fw = (t[T_REL2_W] / t[T_REL3_QZ]) * (1.0f - expf(-t[T_REL3_QZ]));
+    float *v = &vt[12 * (size_t)m];
+    v[0] = eqx; v[1] = t[T_REL1_QX]; v[2] = t[T_REL2_QX]; v[3] = fu;
+    v[4] = t[T_REL1_U]; v[5] = t[T_REL3_U]; v[6] = eqz; v[7] = t[T_REL1_QZ];
+    v[8] = t[T_REL2_QZ]; v[9] = fw; v[10] = t[T_REL1_W]; v[11] = t[T_REL3_W];
+    at[2 * m] = t[T_AUXW];
+    at[2 * m + 1] = t[T_AUXQZ];
+  }
+  HIPCHK(h, hipMemcpyAsync((void *)h->kp.JM, jp.data(), jp.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((void *)h->kp.stab, st.data(), st.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((void *)h->kp.vtab, vt.data(), vt.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((void *)h->kp.atab, at.data(), at.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->model_ok = true;
+  return 0;
+}
+
+static int upload_pml_host(bq_handle *h, const bq::PmlHost &p) {
+  KParams &k = h->kp;
+  for (int t = 0; t < 8; ++t) {
+    HIPCHK(h, hipMemcpyAsync((void *)k.x1[t], p.x1[t].data(), p.x1[t].size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync((void *)k.x2[t], p.x2[t].data(), p.x2[t].size() * 4, hipMemcpyHostToDevice, h->stream));
+  }
+  for (int t = 0; t < 4; ++t) {
+    HIPCHK(h, hipMemcpyAsync((void *)k.z1[t], p.z1[t].data(), p.z1[t].size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync((void *)k.z2[t], p.z2[t].data(), p.z2[t].size() * 4, hipMemcpyHostToDevice, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->pml_ok = true;
+  return 0;
+}
+
+int bq_build_pml(bq_handle *h, float r, float tau, float pow_, float sf, float wc) {
+  if (!h) return -1;
+  if (!h->model_ok) return fail(h, -6, "bq_build_pml needs bq_upload_model first");
+  bq::PmlHost p;
+  bq::build_pml(h->g, h->jm_host.data(), h->tab_host.data(), r, tau, pow_, sf, wc, p);
+  return upload_pml_host(h, p);
+}
+
+// MODULE PML order -> (group, index) of PmlHost: XL/XH 0..7 = a b c d of XL,
+// then XH; ZH 0..3 = a b c d.
+static void pml_slot(int id, int *grp, int *idx) {
+  static const int map[24][2] = {
+      {0, 0}, {0, 1}, {0, 2}, {0, 4}, {0, 5}, {0, 6}, {0, 3}, {0, 7},  // ODA..ODD _XL/_XH
+      {2, 0}, {2, 1}, {2, 2}, {2, 3},                                  // ODA_ZH..ODD_ZH
+      {1, 0}, {1, 1}, {1, 2}, {1, 4}, {1, 5}, {1, 6}, {1, 3}, {1, 7},  // ODA2..ODD2 _XL/_XH
+      {3, 0}, {3, 1}, {3, 2}, {3, 3}};                                 // ODA2_ZH..ODD2_ZH
+  *grp = map[id][0];
+  *idx = map[id][1];
+}
+
+int bq_upload_pml(bq_handle *h, const float *const *coef) {
+  if (!h || !coef) return fail(h, -1, "null argument");
+  const GridDesc &g = h->g;
+  bq::PmlHost p;
+  p.resize(g.T, g.NX, g.NZ);
+  for (int id = 0; id < 24; ++id) {
+    int grp, idx;
+    pml_slot(id, &grp, &idx);
+    std::vector<float> &v = grp == 0 ? p.x1[idx] : grp == 1 ? p.x2[idx] : grp == 2 ? p.z1[idx] : p.z2[idx];
+    if (!coef[id]) return fail(h, -1, "null PML array");
+    std::memcpy(v.data(), coef[id], v.size() * 4);
+  }
+  return upload_pml_host(h, p);
+}
+
+int bq_get_pml(bq_handle *h, int id, float *out) {
+  if (!h || id < 0 || id >= 24) return fail(h, -1, "bad PML id");
+  int grp, idx;
+  pml_slot(id, &grp, &idx);
+  const KParams &k = h->kp;
+  const float *src = grp == 0 ? k.x1[idx] : grp == 1 ? k.x2[idx] : grp == 2 ? k.z1[idx] : k.z2[idx];
+  const GridDesc &g = h->g;
+  const size_t n = grp == 0 ? (size_t)(g.T + 1) * g.NZ : grp == 1 ? (size_t)(g.T + 2) * g.NZ
+                 : grp == 2 ? (size_t)(g.T + 1) * g.NX : (size_t)(g.T + 2) * g.NX;
+  if (out) {
+    HIPCHK(h, hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return (int)n;
+}
+
+int bq_upload_bottom_nr(bq_handle *h, const float abw[8], const float abh[8]) {
+  if (!h || !abw || !abh) return fail(h, -1, "null argument");
+  for (int c = 0; c < 8; ++c) { h->kp.abw[c] = abw[c]; h->kp.abh[c] = abh[c]; }
+  return 0;
+}
+
+int bq_set_source(bq_handle *h, const float *sourtf) {
+  if (!h || !sourtf) return fail(h, -1, "null argument");
+  const int n = h->g.MT2;
+  const float DT = h->g.DT;
+  // STRESS_LIN_NN injects -(S-SOURSP)/DT into TXX, TZZ and +(F-SOURFP)/DT into
+  // PRES, SOURSP/SOURFP being the previous step's values starting from 0
+  // (stress_lin_nn.F90:58-65, alloc_t_seis.F90:69-70).
+  std::vector<float> ds(2 * (size_t)n);
+  float sp = 0.0f, fp = 0.0f;
+  for (int s = 0; s < n; ++s) {
+    const float S = sourtf[2 * (size_t)s], F = sourtf[2 * (size_t)s + 1];
+    ds[2 * (size_t)s] = (S - sp) / DT;
+    ds[2 * (size_t)s + 1] = (F - fp) / DT;
+    sp = S;
+    fp = F;
+  }
+  HIPCHK(h, hipMemcpyAsync((void *)h->kp.dsrc, ds.data(), ds.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->src_ok = true;
+  return 0;
+}
+
+int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *lrecx,
+                     const int32_t *irecz, const int32_t *lrecz) {
+  if (!h || mr < 0 || (mr > 0 && (!irecx || !lrecx || !irecz || !lrecz)))
+    return fail(h, -1, "bad receiver arguments");
+  std::vector<int4> r(mr > 0 ? mr : 1);
+  for (int j = 0; j < mr; ++j) r[j] = make_int4(irecx[j], lrecx[j], irecz[j], lrecz[j]);
+  int4 *rd = nullptr;
+  float *tr = nullptr;
+  if (dalloc(h, &rd, (size_t)(mr > 0 ? mr : 1))) return -101;
+  if (dalloc(h, &tr, (size_t)h->g.MT2 * (mr > 0 ? mr : 1) * 6)) return -101;
+  HIPCHK(h, hipMemcpyAsync(rd, r.data(), r.size() * sizeof(int4), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->kp.rec = rd;
+  h->kp.traces = tr;
+  h->kp.MR = mr;
+  h->mr = mr;
+  return 0;
+}
+
+int bq_enable_timing(bq_handle *h, int on) {
+  if (!h) return -1;
+  h->timing = on != 0;
+  return 0;
+}
+
+int bq_run(bq_handle *h, int itile_begin, int itile_end) {
+  if (!h) return -1;
+  if (!h->model_ok || !h->pml_ok || !h->src_ok)
+    return fail(h, -6, "model, PML and source must be set before bq_run");
+  if (itile_begin != h->next_step) return fail(h, -6, "steps must run in order");
+  if (itile_end > h->g.MT2) return fail(h, -7, "step beyond MT2");
+  if (itile_end < itile_begin) return 0;
+  const KParams &k = h->kp;
+  const GridDesc &g = h->g;
+  const dim3 blk(kBX, kBY);
+  const dim3 gs((g.NX + kBX - 1) / kBX, (g.PMLZH - 1 + kBY - 1) / kBY);  // rows 2..PMLZH
+  const dim3 gv((g.NX + kBX - 1) / kBX, (g.PMLZH - 2 + kBY - 1) / kBY);  // rows 2..PMLZH-1
+  const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
+  const int nsteps = itile_end - itile_begin + 1;
+  if (h->timing) {
+    const size_t need = (size_t)nsteps * 5;
+    while (h->ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(h, hipEventCreate(&e));
+      h->ev.push_back(e);
+    }
+  }
+  for (int it = itile_begin; it <= itile_end; ++it) {
+    hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * 5] : nullptr;
+    if (e) (void)hipEventRecord(e[0], h->stream);
+    hipLaunchKernelGGL(k_stress, gs, blk, 0, h->stream, k, it);
+    if (e) (void)hipEventRecord(e[1], h->stream);
+    hipLaunchKernelGGL(k_velocity, gv, blk, 0, h->stream, k);
+    if (e) (void)hipEventRecord(e[2], h->stream);
+    hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
+    if (e) (void)hipEventRecord(e[3], h->stream);
+    if (k.MR > 0) hipLaunchKernelGGL(k_sample, dim3((k.MR + 255) / 256), dim3(256), 0, h->stream, k, it);
+    if (e) (void)hipEventRecord(e[4], h->stream);
+  }
+  HIPCHK(h, hipGetLastError());
+  h->next_step = itile_end + 1;
+  if (h->timing) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double acc[4] = {0, 0, 0, 0};
+    for (int s = 0; s < nsteps; ++s) {
+      for (int q = 0; q < 4; ++q) {
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[(size_t)s * 5 + q], h->ev[(size_t)s * 5 + q + 1]));
+        acc[q] += ms;
+      }
+    }
+    for (int q = 0; q < 4; ++q) h->ktime[q] = acc[q] / nsteps;
+    h->timed_steps = nsteps;
+  }
+  return 0;
+}
+
+int bq_kernel_times(bq_handle *h, double out[4]) {
+  if (!h || !out) return -1;
+  if (!h->timed_steps) return fail(h, -6, "no timed run");
+  for (int q = 0; q < 4; ++q) out[q] = h->ktime[q];
+  return 0;
+}
+
+int bq_sync(bq_handle *h) {
+  if (!h) return -1;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int bq_get_traces(bq_handle *h, float *out, int first_step, int nsteps) {
+  if (!h || !out) return fail(h, -1, "null argument");
+  if (!h->kp.traces || first_step < 1 || nsteps < 0 || first_step + nsteps - 1 > h->g.MT2)
+    return fail(h, -9, "trace range");
+  HIPCHK(h, hipMemcpyAsync(out, h->kp.traces + (size_t)(first_step - 1) * h->mr * 6,
+                           (size_t)nsteps * h->mr * 6 * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static float *field_ptr(bq_handle *h, int id) {
+  if (id < 0 || id >= 8) return nullptr;
+  return h->fields + (size_t)id * h->P * h->g.NZ;
+}
+
+int bq_get_field(bq_handle *h, int field_id, float *host) {
+  if (!h || !host) return fail(h, -1, "null argument");
+  float *d = field_ptr(h, field_id);
+  if (!d) return fail(h, -8, "bad field id");
+  HIPCHK(h, hipMemcpy2DAsync(host, (size_t)h->g.NX * 4, d, (size_t)h->P * 4, (size_t)h->g.NX * 4,
+                             h->g.NZ, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int bq_set_field(bq_handle *h, int field_id, const float *host) {
+  if (!h || !host) return fail(h, -1, "null argument");
+  float *d = field_ptr(h, field_id);
+  if (!d) return fail(h, -8, "bad field id");
+  HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->P * 4, host, (size_t)h->g.NX * 4, (size_t)h->g.NX * 4,
+                             h->g.NZ, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // extern "C"

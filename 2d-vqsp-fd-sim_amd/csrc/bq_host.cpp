@@ -1,0 +1,171 @@
+// bq_host.cpp - host-side set-up of the Biot time loop (no device work).
+//
+// These are the reference's one-off set-up computations whose outputs the time
+// loop consumes: the PML damping profiles (PPML), the wave speeds they are
+// scaled with (CALCUL_VELOCITY) and the source time function (OPEN_SR_FILE).
+// They run once per run on the host in binary32, in the reference's expression
+// order, so the device sees exactly the coefficients the Fortran host would
+// have produced. Built with -ffp-contract=off: every operation rounds as
+// written.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "bq_internal.h"
+
+namespace bq {
+
+// CALCUL_VELOCITY, calcul_velocity.F90:16-36: eigen-speeds of A^-1 B for the
+// 2x2 Biot system, in the no-friction limit.
+void wave_speeds(float sig1, float p, float ainv, float mu, float rel1, float rel2,
+                 float rel3, float *vpf, float *vps, float *vs) {
+  const float ai[2][2] = {{rel1, -rel3}, {-rel3, rel2}};
+  const float bm[2][2] = {{sig1, p}, {p, ainv}};
+  float tt[2][2];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) tt[r][c] = ai[r][0] * bm[0][c] + ai[r][1] * bm[1][c];
+  const float tr = tt[0][0] + tt[1][1];
+  const float delta = tr * tr + 4.0f * (tt[0][1] * tt[1][0] - tt[0][0] * tt[1][1]);
+  const float sd = sqrtf(delta);
+  *vpf = sqrtf((tr + sd) / 2.0f);
+  *vps = sqrtf((tr - sd) / 2.0f);
+  *vs = sqrtf(rel1 * mu);
+  if (mu == 0.0f) *vs = *vpf;
+}
+
+// One damping profile of PPML (ppml.F90:169-207) for a boundary cell of speed
+// vel: integer nodes j = 0..T into (a,b,c,d) and half-integer nodes
+// j = 0..T+1 into (a2,b2,c2,d2). Node 0 is undamped (a=b=d=1, c=0).
+static void damping_profile(int T, float omg, float vel, float sf, float wc, float pw, float dt,
+                            float *a, float *b, float *c, float *d, float *a2, float *b2,
+                            float *c2, float *d2) {
+  for (int j = 0; j <= T; ++j) { a[j] = 1.0f; b[j] = 1.0f; c[j] = 0.0f; }
+  for (int j = 0; j <= T + 1; ++j) { a2[j] = 1.0f; b2[j] = 1.0f; c2[j] = 0.0f; }
+  auto node = [&](float x, float *pa, float *pb, float *pc) {
+    const float om = omg * vel * powf(x, pw);
+    const float ap = 1.0f + sf * om * 0.075f / 0.91f;
+    const float den = expf(-om / ap - wc * dt);
+    *pa = (den + 1.0f) / 2.0f / ap;
+    *pb = den;
+    *pc = om / (om + wc * ap * dt) * (den - 1.0f);
+  };
+  for (int j = 1; j <= T; ++j) node((float)j, &a[j], &b[j], &c[j]);
+  for (int j = 1; j <= T + 1; ++j) node((float)j - 0.5f, &a2[j], &b2[j], &c2[j]);
+  for (int j = 0; j <= T; ++j) d[j] = (c[j] + 2.0f) / (b[j] + 1.0f);
+  for (int j = 0; j <= T + 1; ++j) d2[j] = (c2[j] + 2.0f) / (b2[j] + 1.0f);
+}
+
+// PPML, ppml.F90:159-306, serial build. Side strips use the S speed of the
+// outermost PML column of each row; the bottom strip uses the fast-P speed of
+// the last row (ppml.F90:168-169, 215-216, 265-266). KTBO != 0 switches the
+// bottom damping off (ppml.F90:261).
+void build_pml(const GridDesc &g, const int32_t *jm, const float *tab, float r, float tau,
+               float pw, float sf, float wc, PmlHost &out) {
+  const int T = g.T, NX = g.NX, NZ = g.NZ;
+  out.resize(T, NX, NZ);
+  float omg = 0.0f;
+  if (T != 0) omg = -tau * logf(r) * g.DT / (float)T / powf((float)T, pw) / g.H;
+  auto tabv = [&](int m, int k) { return tab[(size_t)m * 24 + k]; };
+  for (int L = 0; L < NZ; ++L) {
+    for (int side = 0; side < 2; ++side) {
+      const int m = jm[(size_t)L * NX + (side ? NX - 1 : 0)] - 1;
+      float vpf, vps, vs;
+      wave_speeds(tabv(m, T_SXX1), -tabv(m, T_P1), -tabv(m, T_P3), tabv(m, T_MU),
+                  g.H * tabv(m, T_REL1_U) / g.DT, g.H * tabv(m, T_REL2_QX) / g.DT,
+                  g.H * tabv(m, T_REL3_U) / g.DT, &vpf, &vps, &vs);
+      const size_t o1 = (size_t)L * (T + 1), o2 = (size_t)L * (T + 2);
+      const int base = side ? 4 : 0;  // XL arrays 0..3 (a b c d), XH 4..7
+      damping_profile(T, omg, vs, sf, wc, pw, g.DT, &out.x1[base + 0][o1], &out.x1[base + 1][o1],
+                      &out.x1[base + 2][o1], &out.x1[base + 3][o1], &out.x2[base + 0][o2],
+                      &out.x2[base + 1][o2], &out.x2[base + 2][o2], &out.x2[base + 3][o2]);
+    }
+  }
+  if (g.KTBO != 0) omg = 0.0f;
+  for (int ix = 0; ix < NX; ++ix) {
+    const int m = jm[(size_t)(NZ - 1) * NX + ix] - 1;
+    float vpf, vps, vs;
+    wave_speeds(tabv(m, T_SZZ2), -tabv(m, T_P2), -tabv(m, T_P3), tabv(m, T_MU),
+                g.H * tabv(m, T_REL1_W) / g.DT, g.H * tabv(m, T_REL2_QZ) / g.DT,
+                g.H * tabv(m, T_REL3_W) / g.DT, &vpf, &vps, &vs);
+    const size_t o1 = (size_t)ix * (T + 1), o2 = (size_t)ix * (T + 2);
+    damping_profile(T, omg, vpf, sf, wc, pw, g.DT, &out.z1[0][o1], &out.z1[1][o1],
+                    &out.z1[2][o1], &out.z1[3][o1], &out.z2[0][o2], &out.z2[1][o2],
+                    &out.z2[2][o2], &out.z2[3][o2]);
+  }
+}
+
+// --- OPEN_SR_FILE: clamped cubic spline (open_sr_file.F90:121-226) ----------
+
+static int spl_locate(const std::vector<float> &xx, float x) {
+  const int n = (int)xx.size();
+  const bool asc = xx[n - 1] >= xx[0];
+  int lo = 0, hi = n + 1;
+  while (hi - lo > 1) {
+    const int mid = (hi + lo) / 2;
+    if (asc == (x >= xx[mid - 1])) lo = mid; else hi = mid;
+  }
+  if (x == xx[0]) return 1;
+  if (x == xx[n - 1]) return n - 1;
+  return lo;
+}
+
+}  // namespace bq
+
+extern "C" int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float dt,
+                                  float h, int key_sour, float qr, float *sourtf) {
+  if (!t || !v || !sourtf || nstf < 1 || mt2 < 1) return -1;
+  const int n = nstf + 1;
+  std::vector<float> x(t, t + nstf), y(v, v + nstf);
+  x.push_back((float)mt2 * dt);  // TIME(NSTFS) = MT2*DT
+  y.push_back(y[n - 2]);         // STF(NSTFS) = STF(NSTFS-1)
+  // spline(x, y, yp1 = 0, ypn = 0, y2)
+  std::vector<float> a(n), b(n), c(n), r(n), r0(n), y2(n), gam(n);
+  for (int i = 0; i < n - 1; ++i) c[i] = x[i + 1] - x[i];
+  for (int i = 0; i < n - 1; ++i) r[i] = 6.0f * ((y[i + 1] - y[i]) / c[i]);
+  r0 = r;
+  for (int i = 1; i < n - 1; ++i) r[i] = r0[i] - r0[i - 1];
+  for (int i = 1; i < n - 1; ++i) a[i] = c[i - 1];
+  for (int i = 1; i < n - 1; ++i) b[i] = 2.0f * (c[i] + a[i]);
+  b[0] = 1.0f;
+  b[n - 1] = 1.0f;
+  r[0] = (3.0f / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - 0.0f);
+  c[0] = 0.5f;
+  r[n - 1] = (-3.0f / (x[n - 1] - x[n - 2])) * ((y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]) - 0.0f);
+  a[n - 1] = 0.5f;
+  // tridag: sub-diagonal a(2:n), diagonal b, super-diagonal c(1:n-1)
+  float bet = b[0];
+  if (bet == 0.0f) return -2;
+  y2[0] = r[0] / bet;
+  for (int j = 1; j < n; ++j) {
+    gam[j] = c[j - 1] / bet;
+    bet = b[j] - a[j] * gam[j];
+    if (bet == 0.0f) return -2;
+    y2[j] = (r[j] - a[j] * y2[j - 1]) / bet;
+  }
+  for (int j = n - 2; j >= 0; --j) y2[j] = y2[j] - gam[j + 1] * y2[j + 1];
+  std::memset(sourtf, 0, sizeof(float) * 2 * (size_t)mt2);
+  const float h2 = h * h;
+  for (int I = 1; I <= mt2; ++I) {
+    const float T = (float)(I - 1) * dt;
+    if (T > x[n - 2]) break;  // zero padding after the last sample
+    const int klo = std::max(std::min(bq::spl_locate(x, T), n - 1), 1);
+    const int khi = klo + 1;
+    const float hh = x[khi - 1] - x[klo - 1];
+    if (hh == 0.0f) return -3;
+    const float aa = (x[khi - 1] - T) / hh;
+    const float bb = (T - x[klo - 1]) / hh;
+    const float s = aa * y[klo - 1] + bb * y[khi - 1] +
+                    ((aa * aa * aa - aa) * y2[klo - 1] + (bb * bb * bb - bb) * y2[khi - 1]) *
+                        (hh * hh) / 6.0f;
+    float *o = sourtf + (size_t)(I - 1) * 2;
+    switch (key_sour) {
+      case 0: o[1] = qr * s / h2; o[0] = (1.0f - qr) * s / h2; break;
+      case 1: o[1] = 0.0f; o[0] = s / h2; break;
+      case 2: o[1] = s / h2; o[0] = qr * s / h2; break;
+      case 3: o[1] = s / h2; o[0] = 0.0f; break;
+      default: return -4;
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,39 @@
+// bq_internal.h - shared definitions of the Biot time-loop library.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace bq {
+
+// .MO table columns (read_model_par.F90:52-60).
+enum {
+  T_MU = 0, T_SXX1, T_SXX2, T_SXX3, T_SZZ1, T_SZZ2, T_SZZ3, T_P1, T_P2, T_P3,
+  T_REL1_U, T_REL1_W, T_REL2_U, T_REL2_W, T_REL3_U, T_REL3_W,
+  T_REL1_QX, T_REL1_QZ, T_REL2_QX, T_REL2_QZ, T_REL3_QX, T_REL3_QZ,
+  T_AUXW, T_AUXQZ, T_NTAB
+};
+
+struct GridDesc {
+  int MX, MZ, T, PMLL, PMLXH, PMLZH, NX, NZ, LIN_X, LIN_Z, KTBO, MT2, JMNUM;
+  float H, DT;
+};
+
+// Host copies of the PML damping coefficients. x1: ODA/ODB/ODC/ODD _XL then
+// _XH, each (0:T, NZc) as [l][j]; x2: same with 0:T+1; z1: ODA/ODB/ODC/ODD _ZH
+// (0:T, NXc) as [i][j]; z2: same with 0:T+1.
+struct PmlHost {
+  std::vector<float> x1[8], x2[8], z1[4], z2[4];
+  void resize(int T, int NX, int NZ) {
+    for (auto &v : x1) v.assign((size_t)(T + 1) * NZ, 0.0f);
+    for (auto &v : x2) v.assign((size_t)(T + 2) * NZ, 0.0f);
+    for (auto &v : z1) v.assign((size_t)(T + 1) * NX, 0.0f);
+    for (auto &v : z2) v.assign((size_t)(T + 2) * NX, 0.0f);
+  }
+};
+
+void wave_speeds(float sig1, float p, float ainv, float mu, float rel1, float rel2, float rel3,
+                 float *vpf, float *vps, float *vs);
+void build_pml(const GridDesc &g, const int32_t *jm, const float *tab, float r, float tau,
+               float pw, float sf, float wc, PmlHost &out);
+
+}  // namespace bq

@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput of the Biot time loop on one MI355X.
+
+Workload (the roofline case): layered half-space 8000 x 4000 cells, dx = 2 m,
+10 seeded layers times a seeded Gaussian-correlated field (50 m, 5 % std) on
+porosity, Kdry and mu (quantized to 16 levels per parameter, so the material
+table stays indexed), free surface by stress imaging, 20-cell PML on the other
+sides, 20 Hz Ricker source at the surface, receivers on the surface. A "step"
+is one DO ITILE iteration of TIME_LOOP: stress half-step, velocity half-step,
+edge rows and receiver samples, with all inputs resident in HBM.
+
+Prints one JSON line. ``--gpus N`` with N > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "2d-vqsp-fd-sim_amd"))
+
+from biotfd import formats, hostsetup, lib, models  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+B_STEP = 104          # algorithmic bytes per cell per step (indexed materials)
+B_HALF = 52           # per half-step kernel: 4 fields read, 4 fields read+written, 1 int32 index
+
+
+def make_workload(steps: int, mx=8000, mz=4000):
+    job = models.layered_job(name="ROOF", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=10, seed=2,
+                             tpml=20, f0=20.0, n_rec=1000, field_std=0.05, corr_len=50.0,
+                             levels=16)
+    return job
+
+
+def cpu_reference_baseline(job_full, steps_a=2, steps_b=7, budget_s=240.0):
+    """Times the reference (built from its sources into oracle/_ref/fd2d, serial,
+    one core) on the same grid: per-step time from two runs that differ only in
+    the step count, so set-up and input reading cancel."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "fd2d")
+    if not os.path.exists(exe):
+        return None
+    times = {}
+    tmp = tempfile.mkdtemp(prefix="bqref_", dir=os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None)
+    try:
+        for n in (steps_a, steps_b):
+            job = make_like(job_full, n)
+            d = os.path.join(tmp, f"s{n}")
+            formats.write_job(job, d)
+            t0 = time.perf_counter()
+            subprocess.run([exe], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=budget_s)
+            times[n] = time.perf_counter() - t0
+            print(f"[bench] reference {n} steps: {times[n]:.2f} s", file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    per_step = (times[steps_b] - times[steps_a]) / (steps_b - steps_a)
+    return per_step
+
+
+def make_like(job, steps):
+    import copy
+    j = copy.copy(job)
+    j.mt = float(np.float32((steps - 0.5) * np.float32(job.dt)))
+    j.key_tlv = False
+    j.key_snv = True
+    j.key_tlp = False
+    j.ipas2 = 10 ** 8
+    j.receivers = []
+    assert j.mt2 == steps
+    return j
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--mx", type=int, default=8000)
+    ap.add_argument("--mz", type=int, default=4000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    total = args.warmup + args.steps
+    t0 = time.perf_counter()
+    job = make_workload(total, args.mx, args.mz)
+    nz, nx = hostsetup.grid_shape(job)
+    print(f"[bench] model ready: {nx}x{nz} cells, JMNUM={job.table.shape[0]} "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    sim = lib.Simulation(job)
+    print("[bench] uploaded", file=sys.stderr, flush=True)
+    sim.run(args.warmup)
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    sim.enable_timing(True)
+    sim.sync()
+    t1 = time.perf_counter()
+    sim.run(args.steps, sync=False)
+    sim.sync()
+    t2 = time.perf_counter()
+    ktimes = sim.kernel_times()   # ms per launch: stress, velocity, edges, samples
+    elapsed = t2 - t1
+    ms_step = elapsed * 1e3 / args.steps
+    cells = nx * nz
+    gcell = cells * args.steps / elapsed / 1e9
+    u = sim.field("U")
+    if not np.isfinite(u).all():
+        raise SystemExit("non-finite field after run")
+    dom = int(np.argmax(ktimes[:2]))
+    dom_name = ("k_stress", "k_velocity")[dom]
+    achieved = B_HALF * cells / (ktimes[dom] * 1e-3) / 1e9
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tr_path):
+        with open(tr_path) as f:
+            traffic = json.load(f).get(dom_name)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        per_step = cpu_reference_baseline(job)
+        if per_step and per_step > 0:
+            cpu = {"value": round(cells / per_step / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": f"reference fd2d (flang -O2 -ffp-contract=off, serial) on the same "
+                             f"{nx}x{nz} grid, 7 minus 2 steps, no output"}
+
+    out = {
+        "metric": "Gcell-updates/s & %HBM roofline",
+        "value": round(gcell, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"layered+correlated Biot half-space {args.mx}x{args.mz}, dx=2 m, "
+                               f"TPML=20, free surface (stress imaging)",
+                   "grid_cells": cells, "nxc": nx, "nzc": nz, "jmnum": int(job.table.shape[0]),
+                   "b_alg_bytes_per_cell_step": B_STEP, "receivers": len(job.receivers)},
+        "hbm_roofline_frac_step": round(gcell * 1e9 * B_STEP / (HBM_PEAK_GBS * 1e9), 4),
+        "kernel_ms": {"k_stress": round(ktimes[0], 5), "k_velocity": round(ktimes[1], 5),
+                      "k_edges": round(ktimes[2], 5), "k_sample": round(ktimes[3], 5)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

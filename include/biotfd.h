@@ -1,0 +1,136 @@
+/*
+ * biotfd.h - C ABI of the MI355X time loop for the 2-D Biot velocity-stress
+ * simulator (drop-in for SUBROUTINE TIME_LOOP, time_loop.F90:1-209).
+ *
+ * The reference's driver calls TIME_LOOP with no arguments (main.F90:84;
+ * interface at mod_interfaces.F90:1265) and keeps all state in Fortran modules.
+ * This ABI replaces the body of the step loop (time_loop.F90:91-190) with a
+ * handle that owns the device-resident grid. A Fortran TIME_LOOP replacement
+ * binds these entry points with ISO_C_BINDING (scalars by VALUE, arrays as
+ * assumed-size dummies); see INTEGRATION.md for the interface block.
+ *
+ * Conventions
+ *  - Fortran (x,z) arrays are C [z][x] arrays with the lower bounds removed:
+ *    element (i,l) of a full-grid array sits at l*NXc + (i-PMLL), with
+ *    PMLL = 1-TPML, NXc = MX+2*TPML, NZc = MZ'+TPML+1, MZ' as after
+ *    read_open.F90:149-151 (MZ+2 with stress imaging).
+ *  - Every real is IEEE binary32 (the reference's PP = SP, mod_precision.F90).
+ *  - Calls return 0 on success and a negative code on failure; the message is
+ *    available from bq_last_error. The reference reports errors by writing to
+ *    its log unit and STOPping (tl_alloc.F90:39-42); the Fortran binding does
+ *    the same with this message.
+ *  - One handle owns its device memory and one HIP stream on the device that
+ *    was current at bq_create. Calls on one handle must not overlap; distinct
+ *    handles may run concurrently.
+ */
+#ifndef BIOTFD_H
+#define BIOTFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bq_handle bq_handle;
+
+/* Run description. Fields follow MODULE CONTROL_DATA (mod_contr_data.F90:13-31)
+ * and MODULE AUXIL (mod_auxil.F90:24-33) after READ_OPEN and AUX1. */
+typedef struct bq_desc {
+  int32_t mx;             /* MX: physical cells in x */
+  int32_t mz;             /* MZ' : MZ after read_open.F90:149-151 */
+  int32_t tpml;           /* TPML (1..256) */
+  int32_t stress_imaging; /* STRESS_IMAGING (logical as 0/1); 1 is supported */
+  int32_t lin_x, lin_z;   /* source cell, read_open.F90:163-169 */
+  int32_t ktbo;           /* KTBO: bottom boundary key (0 = PML only) */
+  int32_t mt2;            /* MT2: number of time steps */
+  int32_t jmnum;          /* JMNUM: entries in the material table */
+  int32_t reserved;       /* must be 0 */
+  double h, dt;           /* H, DT (binary32 values widened to double) */
+} bq_desc;
+
+/* Field ids for bq_get_field / bq_set_field (mod_wavefield.F90:20,
+ * mod_stressfield.F90:8). */
+enum {
+  BQ_U = 0, BQ_W = 1, BQ_QX = 2, BQ_QZ = 3,
+  BQ_TXX = 4, BQ_TZZ = 5, BQ_TXZ = 6, BQ_PRES = 7
+};
+
+/* Allocates and zeroes the grid (TL_ALLOC, tl_alloc.F90:24-56; PML memories
+ * as ppml.F90:115-153; bottom history as alloc_nr.F90:41-52). */
+int bq_create(const bq_desc *desc, bq_handle **out);
+
+/* READ_MODEL + READ_MODEL_PAR results (read_model.F90:57-87,
+ * read_model_par.F90:37-74): jm is int32 [NZc][NXc] (1-based material ids,
+ * already replicated into PML and image rows), tab is float [JMNUM][24] in the
+ * .MO order, with REL* entries already scaled by DT/H or DT. The friction
+ * factors EXP(-REL3_Q*) and (REL2/REL3_Q)*(1-EXP(-REL3_Q)) are formed here on
+ * the host in binary32, in the reference's expression order
+ * (inh_qx_int.F90:44, inh_v_uv_int.F90:46). */
+int bq_upload_model(bq_handle *h, const int32_t *jm, const float *tab);
+
+/* PPML (ppml.F90:159-306): builds the 24 damping-coefficient arrays from the
+ * uploaded model, in binary32, and uploads them. Needs bq_upload_model first. */
+int bq_build_pml(bq_handle *h, float r, float tau, float pow_, float sf, float wc);
+
+/* Alternative to bq_build_pml: the 24 arrays as the Fortran host computed
+ * them, in MODULE PML order (mod_pml.F90:7-19): ODA_XL ODB_XL ODC_XL ODA_XH
+ * ODB_XH ODC_XH ODD_XL ODD_XH (each (0:T, NZc)), ODA_ZH ODB_ZH ODC_ZH ODD_ZH
+ * ((0:T, NXc)), ODA2_XL ODB2_XL ODC2_XL ODA2_XH ODB2_XH ODC2_XH ODD2_XL ODD2_XH
+ * ((0:T+1, NZc)), ODA2_ZH ODB2_ZH ODC2_ZH ODD2_ZH ((0:T+1, NXc)). */
+int bq_upload_pml(bq_handle *h, const float *const *coef);
+
+/* Copies one of the 24 PML coefficient arrays back (same order); returns the
+ * element count, or a negative code. out may be NULL to query the size. */
+int bq_get_pml(bq_handle *h, int id, float *out);
+
+/* Bottom non-reflecting coefficients ABW01..ABW22 and ABH01..ABH22
+ * (mod_nr_bound.F90:47-52, PRE_NR_COEF1 pre_nr_coef1.F90:185-201), each in the
+ * order 01 02 10 11 12 20 21 22. All zero for KTBO = 0. */
+int bq_upload_bottom_nr(bq_handle *h, const float abw[8], const float abh[8]);
+
+/* SOURTF_A(2,MT2) (open_sr_file.F90:78-110): per step, solid then fluid. */
+int bq_set_source(bq_handle *h, const float *sourtf);
+
+/* Receivers (read_open.F90:234-236): 1-based x index and z index of the U
+ * node (IRECX, LRECX) and of the W node (IRECZ, LRECZ). Allocates a device
+ * trace buffer for MT2 steps. */
+int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *lrecx,
+                     const int32_t *irecz, const int32_t *lrecz);
+
+/* Advances steps itile_begin..itile_end (1-based, DO ITILE of
+ * time_loop.F90:65); steps must follow one another. Returns after queuing;
+ * bq_sync waits. */
+int bq_run(bq_handle *h, int itile_begin, int itile_end);
+int bq_sync(bq_handle *h);
+
+/* Receiver samples of steps first_step .. first_step+nsteps-1, [step][mr][6]:
+ * UM(IRECX,LRECX), -WM(IRECZ,LRECZ), QX(IRECX,LRECX), -QZ(IRECZ,LRECZ),
+ * PRES(IRECZ,LRECX), -TXX(IRECZ,LRECX) (store_seismograms.F90:58-83,
+ * store_seismograms_pressure.F90:50-56). Out-of-grid receivers read 0. */
+int bq_get_traces(bq_handle *h, float *out, int first_step, int nsteps);
+
+/* Whole-grid copy of one field, [NZc][NXc] (snapshots, tests, restart). */
+int bq_get_field(bq_handle *h, int field_id, float *host);
+int bq_set_field(bq_handle *h, int field_id, const float *host);
+
+/* Host-side set-up helpers (no handle, no device work). */
+/* OPEN_SR_FILE (open_sr_file.F90:41-116): clamped cubic spline of the nstf
+ * STF.DAT samples onto t=(n-1)*DT, scaled by KEY_SOUR/QR/H, -> sourtf[2*mt2]. */
+int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float dt, float h,
+                       int key_sour, float qr, float *sourtf);
+
+/* Informational. */
+const char *bq_last_error(bq_handle *h);
+int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
+/* Average device time of the last bq_run's launches per kernel, ms: out[0]
+ * stress, out[1] velocity, out[2] edge rows, out[3] receivers. Timing must
+ * have been enabled with bq_enable_timing(h, 1) before that run. */
+int bq_enable_timing(bq_handle *h, int on);
+int bq_kernel_times(bq_handle *h, double out[4]);
+void bq_destroy(bq_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIOTFD_H */

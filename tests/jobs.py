@@ -1,0 +1,39 @@
+"""Small seeded jobs shared by the tests (sizes the CPU oracle runs in seconds)."""
+import numpy as np
+
+from biotfd import models
+from biotfd.formats import Job
+
+
+def tiny_layered(steps=120, tpml=8, seed=3, mx=48, mz=36, f0=60.0) -> Job:
+    """Three random layers, surface source: waves reach the side PML and the surface."""
+    return models.layered_job(name="TINY", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=3, seed=seed,
+                              tpml=tpml, f0=f0, n_rec=10)
+
+
+def tiny_homog(steps=120, tpml=6, mx=40, mz=30, f0=80.0) -> Job:
+    """Homogeneous model, centred explosive source, receivers on a row above it."""
+    rock = models.Rock(phi=0.2, kdry=10e9, mu=9e9, perm=1e-12)
+    jm = np.ones((mz + 1, mx), dtype=np.int32)
+    recs = models.receivers_row(8, 4, 4, 8)
+    return models.make_job("HOMOG", [rock], jm, 1.0, steps, f0, mx / 2.0, mz / 2.0, recs,
+                           tpml=tpml, key_tlp=True)
+
+
+def tiny_edges(steps=150, tpml=5, mx=36, mz=24, f0=90.0, seed=7) -> Job:
+    """Source in the left PML strip near the bottom row (LIN_Z = MZ'-2), so the
+    bottom PML, the last row and the left strip all see large amplitudes;
+    receivers in the PML, at the surface and out of the grid."""
+    rng = np.random.default_rng(seed)
+    rocks = models.layered_rocks(2, rng)
+    jm = np.ones((mz + 1, mx), dtype=np.int32)
+    jm[mz // 2:, :] = 2
+    jm[:, mx // 2:] = np.where(jm[:, mx // 2:] == 1, 2, 1)
+    h = 1.5
+    xs, zs = 0.0, (mz + 2 - 2 - 2) * h   # LIN_X = 0 (left PML), LIN_Z = MZ' - 2
+    recs = [(-tpml + 2, 3, -tpml + 2, 3), (0, 2, 0, 2), (mx + tpml - 1, mz + 2 + tpml - 1, mx, mz),
+            (5, 1, 5, 1), (10, 0, 10, 0), (mx + tpml + 4, 3, 3, mz + tpml + 9)]
+    return models.make_job("EDGES", rocks, jm, h, steps, f0, xs, zs, recs, tpml=tpml, key_tlp=True)
+
+
+ALL = {"layered": tiny_layered, "homog": tiny_homog, "edges": tiny_edges}
