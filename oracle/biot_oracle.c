@@ -1151,3 +1151,46 @@ EXPORT int oracle_source_from_stf(const real *t, const real *v, int nstf, int mt
   free(x); free(y); free(y2);
   return rc;
 }
+
+/* ====================================================== bottom boundary keys */
+
+/* PRE_NR_COEF1 bottom part (pre_nr_coef1.F90:185-201) with NONREF
+ * (nonref.F90:69-204) for the closed-form keys 0, 2, 4 and 5: wave speeds of
+ * the cell JM(PMLL, MZ'), ABW from the fast-P speed, ABH from the S speed.
+ * Returns -1 for keys whose coefficients come from HIGDON_FD/OptAbsBc. */
+static int nonref_key(int key, real H, real DT, real VEL, real *ax) {
+  for (int k = 0; k < 8; ++k) ax[k] = 0;
+  switch (key) {
+    case 0: return 0;
+    case 2: {
+      real TMP = -DT * VEL / H;
+      ax[2] = (real)1 + TMP; ax[3] = (real)1 - TMP; ax[6] = (real)-1 - TMP; ax[7] = TMP;
+      return 0;
+    }
+    case 4: {
+      real TMP1 = (DT - H / VEL) / (DT + H / VEL);
+      real TMP2 = (real)2 * H / VEL / (DT + H / VEL);
+      ax[0] = TMP1; ax[2] = TMP2; ax[3] = TMP2; ax[5] = TMP1; ax[6] = (real)-1;
+      return 0;
+    }
+    case 5: {
+      real TMP1 = VEL * DT / H;
+      ax[0] = -((real)1 - TMP1) / ((real)1 + TMP1);
+      ax[2] = ((real)1 - TMP1) / ((real)1 + TMP1);
+      ax[3] = (real)1;
+      return 0;
+    }
+    default: return -1;
+  }
+}
+
+EXPORT int oracle_bottom_nr(orc *o, int key, real *abw, real *abh) {
+  int jm = JMF(o->PMLL, o->MZ);
+  real VPF, VPS, VS;
+  calcul_velocity(TAB(T_SXX1, jm), -TAB(T_P1, jm), -TAB(T_P3, jm), TAB(T_MU, jm),
+                  o->H * TAB(T_REL1_U, jm) / o->DT, o->H * TAB(T_REL2_QX, jm) / o->DT,
+                  o->H * TAB(T_REL3_U, jm) / o->DT, &VPF, &VPS, &VS);
+  if (nonref_key(key, o->H, o->DT, VPF, abw)) return -1;
+  if (nonref_key(key, o->H, o->DT, VS, abh)) return -1;
+  return 0;
+}

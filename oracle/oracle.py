@@ -58,6 +58,7 @@ def load(precision: str = "f32"):
         ("oracle_set_field", [p, C.c_int, p]),
         ("oracle_get_traces", [p, p, C.c_int, C.c_int]),
         ("oracle_get_pml", [p, C.c_int, p]),
+        ("oracle_bottom_nr", [p, C.c_int, p, p]),
         ("oracle_source_from_stf", [p, p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                     C.c_double, p]),
     ):
@@ -107,8 +108,13 @@ class OracleSim:
         self.tab = np.ascontiguousarray(hostsetup.scaled_table(job, self.dtype))
         self._chk(self.lib.oracle_upload_model(h, _ptr(self.jm), _ptr(self.tab)))
         self._chk(self.lib.oracle_build_pml(h, job.r, job.tau, job.pow, job.sf, job.wc))
+        if abw is None and abh is None and job.ktbo != 0:
+            abw = np.zeros(8, self.dtype)
+            abh = np.zeros(8, self.dtype)
+            self._chk(self.lib.oracle_bottom_nr(h, job.ktbo, _ptr(abw), _ptr(abh)))
         abw = np.zeros(8, self.dtype) if abw is None else np.asarray(abw, self.dtype)
         abh = np.zeros(8, self.dtype) if abh is None else np.asarray(abh, self.dtype)
+        self.abw, self.abh = abw, abh
         self._chk(self.lib.oracle_set_bottom_nr(h, _ptr(abw), _ptr(abh)))
         self.sourtf = source_from_stf(job, precision)
         self._chk(self.lib.oracle_set_source(h, _ptr(self.sourtf)))

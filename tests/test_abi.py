@@ -1,0 +1,54 @@
+"""C ABI checks that need no GPU: symbols, host-side helpers."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import jobs
+import oracle
+from biotfd import lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "biotfd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bq_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 19
+    so = lib.load()
+    for n in names:
+        assert hasattr(so, n), n
+    assert set(lib.EXPORTS) <= set(names)
+
+
+@pytest.mark.parametrize("name", sorted(jobs.ALL))
+@pytest.mark.parametrize("key_sour,qr", [(0, 0.0), (0, 0.25), (1, 0.0), (2, 0.4), (3, 0.0)])
+def test_source_spline_matches_oracle(name, key_sour, qr):
+    """OPEN_SR_FILE restated in the library's host code vs the oracle, bit for bit."""
+    job = jobs.ALL[name]()
+    job.key_sour, job.qr = key_sour, qr
+    a = lib.source_from_stf(job)
+    b = oracle.source_from_stf(job, "f32")
+    np.testing.assert_array_equal(a, b)
+    assert np.abs(a).max() > 0
+
+
+def test_create_rejects_bad_descriptions():
+    import ctypes as C
+    so = lib.load()
+    h = C.c_void_p()
+    bad = [
+        lib.Desc(4, 20, 8, 1, 2, 2, 0, 10, 1, 0, 1.0, 1e-4),     # MX too small
+        lib.Desc(40, 30, 0, 1, 2, 2, 0, 10, 1, 0, 1.0, 1e-4),    # TPML 0
+        lib.Desc(40, 30, 8, 0, 2, 2, 0, 10, 1, 0, 1.0, 1e-4),    # no stress imaging
+        lib.Desc(40, 30, 8, 1, 2, 29, 0, 10, 1, 0, 1.0, 1e-4),   # source row past MZ'-2
+        lib.Desc(40, 30, 8, 1, 60, 5, 0, 10, 1, 0, 1.0, 1e-4),   # source column outside
+    ]
+    for d in bad:
+        assert so.bq_create(C.byref(d), C.byref(h)) < 0

@@ -1,10 +1,15 @@
 """HIP path vs the CPU oracle, bit for bit, through the C ABI (needs a GPU)."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import jobs
 import oracle
 from biotfd import lib
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +52,7 @@ def test_bottom_nr_and_upload_pml():
     """KTBO != 0 path: explicit bottom coefficients, PML arrays uploaded from the host."""
     job = jobs.tiny_layered(steps=80)
     job.ktbo = 2
+    job.wc = 40.0   # with KTBO != 0 the bottom damping is off and WC > 0 keeps ODC finite
     abw = np.array([0, 0, 0.9, 1.1, 0, 0, -1.1, 0.1], np.float32)
     abh = np.array([0.05, 0, 0.8, 1.2, 0, 0, -1.2, 0.2], np.float32)
     ref = oracle.OracleSim(job, "f32", abw=abw, abh=abh)
@@ -55,3 +61,30 @@ def test_bottom_nr_and_upload_pml():
     ref.run(job.mt2)
     sim.run(job.mt2)
     _compare(sim, ref, "ktbo")
+    assert np.isfinite(sim.field("U")).all()
+
+
+def _golden_names():
+    import make_golden
+    return make_golden.names()
+
+
+@pytest.mark.parametrize("name", _golden_names())
+def test_bitexact_vs_reference_snapshots(name):
+    """HIP path vs the snapshots the reference itself wrote (tests/golden)."""
+    import make_golden
+    job, z = make_golden.load(name)
+    print(f"golden case {name}", flush=True)
+    abw = abh = None
+    if job.ktbo:
+        o = oracle.OracleSim(job, "f32")
+        abw, abh = o.abw, o.abh
+    sim = lib.Simulation(job, abw=abw, abh=abh)
+    done = 0
+    for k, step in enumerate(z["snap_steps"]):
+        sim.run(int(step) - done)
+        done = int(step)
+        for f, fname in enumerate(("U", "W", "QX", "QZ")):
+            got, ref = sim.field(fname), z["snaps"][k, f]
+            bad = np.argwhere(got != ref)
+            assert bad.size == 0, f"{name} step {step} {fname}: {len(bad)} cells differ"
