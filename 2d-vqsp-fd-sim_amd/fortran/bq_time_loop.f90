@@ -1,0 +1,200 @@
+!=======================================================================
+! TIME_LOOP replacement that runs the step loop on the MI355X.
+!
+! Drop-in for SUBROUTINE TIME_LOOP (time_loop.F90:1-209): same name, no
+! arguments, same module state. The set-up the reference driver already did
+! (READ_MODEL, READ_MODEL_PAR, OPEN_SR_FILE, PRE_NR_COEF1, PPML) is handed to
+! the device once; the steps run there; the reference's own writers
+! STORE_SEISMOGRAMS, STORE_SEISMOGRAMS_PRESSURE and STORE_SNAPSHOTS produce the
+! output files unchanged. Receiver samples come back once per chunk of steps
+! and are placed at the receiver nodes of the host arrays before each writer
+! call, so the writers see exactly the values they would have read.
+! The STOP file (time_loop.F90:79-86) is polled at the start of each chunk.
+!=======================================================================
+SUBROUTINE  TIME_LOOP
+
+  USE ISO_C_BINDING
+  USE BQ_BINDING
+  USE PRECISION   , ONLY: PP
+  USE CONTROL_DATA, ONLY: KEY_SNV, KEY_TLV, KEY_TLP, KEY_TLQ, MX, MZ, MT1, MT2, DT, IPAS2, &
+                          TPML, LIN_X, LIN_Z, STRESS_IMAGING, KTBO, MR,                   &
+                          IRECX, LRECX, IRECZ, LRECZ
+  USE GRID_MEDIUM , ONLY: JM, JMNUM, H, MU,                                             &
+                          SIG_XX_1, SIG_XX_2, SIG_XX_3, SIG_ZZ_1, SIG_ZZ_2, SIG_ZZ_3,   &
+                          PRES_1, PRES_2, PRES_3, REL1_U, REL2_U, REL3_U,               &
+                          REL1_W, REL2_W, REL3_W, REL1_QX, REL2_QX, REL3_QX,            &
+                          REL1_QZ, REL2_QZ, REL3_QZ, AUXIL_W, AUXIL_QZ
+  USE T_LOOP      , ONLY: ITILE
+  USE AUXIL       , ONLY: SOURTF_A, X_MIN, X_MAX, Z_MIN, Z_MAX
+  USE PML         , ONLY: ODA_XL, ODB_XL, ODC_XL, ODA_XH, ODB_XH, ODC_XH, ODD_XL, ODD_XH, &
+                          ODA_ZH, ODB_ZH, ODC_ZH, ODD_ZH,                                 &
+                          ODA2_XL, ODB2_XL, ODC2_XL, ODA2_XH, ODB2_XH, ODC2_XH,           &
+                          ODD2_XL, ODD2_XH, ODA2_ZH, ODB2_ZH, ODC2_ZH, ODD2_ZH
+  USE NONREF_BOUND, ONLY: ABW01, ABW02, ABW10, ABW11, ABW12, ABW20, ABW21, ABW22,         &
+                          ABH01, ABH02, ABH10, ABH11, ABH12, ABH20, ABH21, ABH22
+  USE WAVEFIELD   , ONLY: UM, WM, QX, QZ
+  USE STRESSFIELD , ONLY: TXX, PRES
+  USE INTERFACES  , ONLY: STORE_SEISMOGRAMS, STORE_SEISMOGRAMS_PRESSURE, STORE_SNAPSHOTS
+
+  IMPLICIT NONE
+
+  INTERFACE
+    SUBROUTINE     TL_ALLOC
+    END SUBROUTINE TL_ALLOC
+  END INTERFACE
+
+  INTEGER, PARAMETER :: CHUNK_MAX = 256
+
+  TYPE(BQ_DESC) :: D
+  TYPE(C_PTR)   :: HB
+  TYPE(C_PTR)   :: COEF(24)
+  REAL(C_FLOAT), ALLOCATABLE, TARGET :: PBUF(:)
+  REAL(C_FLOAT), ALLOCATABLE :: TAB(:,:), TR(:,:,:)
+  REAL(C_FLOAT) :: ABW(8), ABH(8)
+  INTEGER       :: RC, I0, I1, J, S, K, OFF, N
+  LOGICAL       :: EX, SEIS
+
+  CALL TL_ALLOC
+
+  D%MX = MX
+  D%MZ = MZ
+  D%TPML = TPML
+  D%STRESS_IMAGING = MERGE(1, 0, STRESS_IMAGING)
+  D%LIN_X = LIN_X
+  D%LIN_Z = LIN_Z
+  D%KTBO = KTBO
+  D%MT2 = MT2
+  D%JMNUM = JMNUM
+  D%RESERVED = 0
+  D%H = REAL(H, C_DOUBLE)
+  D%DT = REAL(DT, C_DOUBLE)
+  RC = BQ_CREATE(D, HB)
+  IF (RC /= 0) CALL FAIL('BQ_CREATE', RC)
+
+  ! material table in the .MO order, as READ_MODEL_PAR left it (scaled)
+  ALLOCATE (TAB(24, JMNUM))
+  TAB( 1,:) = MU;       TAB( 2,:) = SIG_XX_1; TAB( 3,:) = SIG_XX_2; TAB( 4,:) = SIG_XX_3
+  TAB( 5,:) = SIG_ZZ_1; TAB( 6,:) = SIG_ZZ_2; TAB( 7,:) = SIG_ZZ_3
+  TAB( 8,:) = PRES_1;   TAB( 9,:) = PRES_2;   TAB(10,:) = PRES_3
+  TAB(11,:) = REL1_U;   TAB(12,:) = REL1_W;   TAB(13,:) = REL2_U
+  TAB(14,:) = REL2_W;   TAB(15,:) = REL3_U;   TAB(16,:) = REL3_W
+  TAB(17,:) = REL1_QX;  TAB(18,:) = REL1_QZ;  TAB(19,:) = REL2_QX
+  TAB(20,:) = REL2_QZ;  TAB(21,:) = REL3_QX;  TAB(22,:) = REL3_QZ
+  TAB(23,:) = AUXIL_W;  TAB(24,:) = AUXIL_QZ
+  RC = BQ_UPLOAD_MODEL(HB, JM, TAB)
+  IF (RC /= 0) CALL FAIL('BQ_UPLOAD_MODEL', RC)
+  DEALLOCATE (TAB)
+
+  ! the 24 PPML arrays, in MODULE PML order, packed into one buffer
+  N = 8*SIZE(ODA_XL) + 4*SIZE(ODA_ZH) + 8*SIZE(ODA2_XL) + 4*SIZE(ODA2_ZH)
+  ALLOCATE (PBUF(N))
+  OFF = 1
+  CALL PUT(ODA_XL, 1);  CALL PUT(ODB_XL, 2);  CALL PUT(ODC_XL, 3)
+  CALL PUT(ODA_XH, 4);  CALL PUT(ODB_XH, 5);  CALL PUT(ODC_XH, 6)
+  CALL PUT(ODD_XL, 7);  CALL PUT(ODD_XH, 8)
+  CALL PUT(ODA_ZH, 9);  CALL PUT(ODB_ZH, 10); CALL PUT(ODC_ZH, 11); CALL PUT(ODD_ZH, 12)
+  CALL PUT(ODA2_XL, 13); CALL PUT(ODB2_XL, 14); CALL PUT(ODC2_XL, 15)
+  CALL PUT(ODA2_XH, 16); CALL PUT(ODB2_XH, 17); CALL PUT(ODC2_XH, 18)
+  CALL PUT(ODD2_XL, 19); CALL PUT(ODD2_XH, 20)
+  CALL PUT(ODA2_ZH, 21); CALL PUT(ODB2_ZH, 22); CALL PUT(ODC2_ZH, 23); CALL PUT(ODD2_ZH, 24)
+  RC = BQ_UPLOAD_PML(HB, COEF)
+  IF (RC /= 0) CALL FAIL('BQ_UPLOAD_PML', RC)
+  DEALLOCATE (PBUF)
+
+  ABW = [ABW01, ABW02, ABW10, ABW11, ABW12, ABW20, ABW21, ABW22]
+  ABH = [ABH01, ABH02, ABH10, ABH11, ABH12, ABH20, ABH21, ABH22]
+  RC = BQ_UPLOAD_BOTTOM_NR(HB, ABW, ABH)
+  IF (RC /= 0) CALL FAIL('BQ_UPLOAD_BOTTOM_NR', RC)
+
+  RC = BQ_SET_SOURCE(HB, SOURTF_A)
+  IF (RC /= 0) CALL FAIL('BQ_SET_SOURCE', RC)
+
+  SEIS = KEY_TLV .AND. MR > 0
+  IF (SEIS) THEN
+    RC = BQ_SET_RECEIVERS(HB, MR, IRECX, LRECX, IRECZ, LRECZ)
+    IF (RC /= 0) CALL FAIL('BQ_SET_RECEIVERS', RC)
+    ALLOCATE (TR(6, MR, CHUNK_MAX))
+  END IF
+
+  I0 = MT1
+  DO WHILE (I0 <= MT2)
+    INQUIRE ( FILE = 'STOP', EXIST = EX )
+    IF ( EX ) THEN
+      OPEN  ( 35, FILE='STOP', STATUS='UNKNOWN')
+      WRITE ( 35,* ) I0
+      CLOSE ( 35 )
+      CALL BQ_DESTROY(HB)
+      STOP
+    END IF
+    I1 = MIN(MT2, I0 + CHUNK_MAX - 1)
+    IF (KEY_SNV) I1 = MIN(I1, ((I0 + IPAS2 - 1) / IPAS2) * IPAS2)
+    RC = BQ_RUN(HB, I0, I1)
+    IF (RC /= 0) CALL FAIL('BQ_RUN', RC)
+    IF (SEIS) THEN
+      RC = BQ_GET_TRACES(HB, TR, I0, I1 - I0 + 1)
+      IF (RC /= 0) CALL FAIL('BQ_GET_TRACES', RC)
+    ELSE
+      RC = BQ_SYNC(HB)
+      IF (RC /= 0) CALL FAIL('BQ_SYNC', RC)
+    END IF
+    DO ITILE = I0, I1
+      WRITE (11,'(I6)') ITILE
+      IF (SEIS) THEN
+        S = ITILE - I0 + 1
+        DO J = 1, MR
+          IF (INSIDE(IRECX(J), LRECX(J))) THEN
+            UM(IRECX(J), LRECX(J)) =  TR(1, J, S)
+            QX(IRECX(J), LRECX(J)) =  TR(3, J, S)
+          END IF
+          IF (INSIDE(IRECZ(J), LRECZ(J))) THEN
+            WM(IRECZ(J), LRECZ(J)) = -TR(2, J, S)
+            QZ(IRECZ(J), LRECZ(J)) = -TR(4, J, S)
+          END IF
+          IF (INSIDE(IRECZ(J), LRECX(J))) THEN
+            PRES(IRECZ(J), LRECX(J)) =  TR(5, J, S)
+            TXX (IRECZ(J), LRECX(J)) = -TR(6, J, S)
+          END IF
+        END DO
+        CALL STORE_SEISMOGRAMS(ITILE)
+        IF (KEY_TLP) CALL STORE_SEISMOGRAMS_PRESSURE(ITILE)
+      END IF
+    END DO
+    IF (KEY_SNV .AND. MOD(I1, IPAS2) == 0) THEN
+      RC = BQ_GET_FIELD(HB, BQ_U, UM)
+      IF (RC == 0) RC = BQ_GET_FIELD(HB, BQ_W, WM)
+      IF (RC == 0 .AND. KEY_TLQ) RC = BQ_GET_FIELD(HB, BQ_QX, QX)
+      IF (RC == 0 .AND. KEY_TLQ) RC = BQ_GET_FIELD(HB, BQ_QZ, QZ)
+      IF (RC /= 0) CALL FAIL('BQ_GET_FIELD', RC)
+      CALL STORE_SNAPSHOTS(I1)
+    END IF
+    I0 = I1 + 1
+  END DO
+
+  CALL BQ_DESTROY(HB)
+
+CONTAINS
+
+  SUBROUTINE PUT(A, K)
+    REAL(PP), INTENT(IN) :: A(:,:)
+    INTEGER, INTENT(IN)  :: K
+    PBUF(OFF:OFF+SIZE(A)-1) = RESHAPE(A, [SIZE(A)])
+    COEF(K) = C_LOC(PBUF(OFF))
+    OFF = OFF + SIZE(A)
+  END SUBROUTINE PUT
+
+  LOGICAL FUNCTION INSIDE(I, L)
+    INTEGER, INTENT(IN) :: I, L
+    INSIDE = X_MIN <= I .AND. I <= X_MAX .AND. Z_MIN <= L .AND. L <= Z_MAX
+  END FUNCTION INSIDE
+
+  ! Error path of the reference: message to the log unit, then STOP
+  ! (tl_alloc.F90:39-42).
+  SUBROUTINE FAIL(WHAT, CODE)
+    CHARACTER(LEN=*), INTENT(IN) :: WHAT
+    INTEGER, INTENT(IN) :: CODE
+    WRITE (11,*) '  ', WHAT, ' FAILED, CODE= ', CODE, ' ', TRIM(BQ_MESSAGE(HB))
+    WRITE (*,*)  '  ', WHAT, ' FAILED, CODE= ', CODE, ' ', TRIM(BQ_MESSAGE(HB))
+    STOP
+  END SUBROUTINE FAIL
+
+END SUBROUTINE TIME_LOOP
