@@ -21,7 +21,7 @@ EXPORTS = (
     "bq_create", "bq_destroy", "bq_upload_model", "bq_build_pml", "bq_upload_pml", "bq_get_pml",
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
-    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times",
+    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_probe_copy",
 )
 
 
@@ -70,6 +70,7 @@ def load(path: str = LIBPATH):
         "bq_grid_shape": ([p, p, p], C.c_int),
         "bq_enable_timing": ([p, C.c_int], C.c_int),
         "bq_kernel_times": ([p, p], C.c_int),
+        "bq_probe_copy": ([C.c_longlong, C.c_int, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -77,6 +78,15 @@ def load(path: str = LIBPATH):
         f.restype = res
     _lib = lib
     return lib
+
+
+def probe_copy(nfloats: int, iters: int = 20) -> float:
+    """Read+write GB/s of a plain device-to-device copy of nfloats floats."""
+    out = np.zeros(1, dtype=np.float64)
+    rc = load().bq_probe_copy(int(nfloats), int(iters), out.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise BiotError(f"bq_probe_copy failed ({rc})")
+    return float(out[0])
 
 
 def _ptr(a: np.ndarray):

@@ -457,6 +457,15 @@ __global__ void k_sample(KParams k, int itile) {
 
 #undef FA
 
+// Plain streaming copy, used to calibrate HBM traffic counters and to measure
+// the achievable copy bandwidth of the device.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a,
+                                                   float4 *__restrict__ b, size_t n4) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
+       t += (size_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
+}
+
 }  // namespace
 
 // ===================================================================== host
@@ -840,6 +849,32 @@ int bq_kernel_times(bq_handle *h, double out[4]) {
   if (!h || !out) return -1;
   if (!h->timed_steps) return fail(h, -6, "no timed run");
   for (int q = 0; q < 4; ++q) out[q] = h->ktime[q];
+  return 0;
+}
+
+int bq_probe_copy(long long nfloats, int iters, double *gbps) {
+  if (nfloats < 4 || iters < 1 || !gbps) return -1;
+  const size_t n4 = (size_t)nfloats / 4;
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n4 * 16) != hipSuccess) return -101;
+  if (hipMalloc(&b, n4 * 16) != hipSuccess) { (void)hipFree(a); return -101; }
+  (void)hipMemset(a, 0, n4 * 16);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int blocks = 256 * 16;
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < iters; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * n4 * 16 * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
 

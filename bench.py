@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--mx", type=int, default=8000)
     ap.add_argument("--mz", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a plain 1 GiB device copy (k_copy_probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -126,6 +128,10 @@ def main():
         with open(tr_path) as f:
             traffic = json.load(f).get(dom_name)
 
+    copy_gbs = None
+    if args.copy_probe:
+        copy_gbs = lib.probe_copy(1 << 28, 20)
+
     cpu = None
     if not args.no_cpu_baseline:
         per_step = cpu_reference_baseline(job)
@@ -155,6 +161,8 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if copy_gbs is not None:
+        out["copy_probe_gbs"] = round(copy_gbs, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -130,6 +130,12 @@ int bq_enable_timing(bq_handle *h, int on);
 int bq_kernel_times(bq_handle *h, double out[4]);
 void bq_destroy(bq_handle *h);
 
+/* Diagnostic with no counterpart in the reference: streams nfloats binary32
+ * values from one device buffer to another iters times (kernel k_copy_probe)
+ * and reports read+write GB/s. Used to measure the achievable HBM bandwidth
+ * and to calibrate hardware traffic counters. */
+int bq_probe_copy(long long nfloats, int iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif
