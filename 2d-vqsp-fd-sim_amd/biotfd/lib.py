@@ -21,7 +21,7 @@ EXPORTS = (
     "bq_create", "bq_destroy", "bq_upload_model", "bq_build_pml", "bq_upload_pml", "bq_get_pml",
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
-    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_probe_copy",
+    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells", "bq_probe_copy",
 )
 
 
@@ -71,6 +71,7 @@ def load(path: str = LIBPATH):
         "bq_enable_timing": ([p, C.c_int], C.c_int),
         "bq_kernel_times": ([p, p], C.c_int),
         "bq_probe_copy": ([C.c_longlong, C.c_int, p], C.c_int),
+        "bq_kernel_cells": ([p, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -162,10 +163,18 @@ class Simulation:
     def enable_timing(self, on: bool = True):
         self._chk(self.lib.bq_enable_timing(self.h, int(on)))
 
-    def kernel_times(self) -> np.ndarray:
+    def kernel_times(self) -> dict:
+        """ms per step of each kernel: k_stress, k_velocity, k_edges, k_sample."""
         out = np.zeros(4, dtype=np.float64)
         self._chk(self.lib.bq_kernel_times(self.h, _ptr(out)))
-        return out
+        return dict(zip(("k_stress", "k_velocity", "k_edges", "k_sample"), map(float, out)))
+
+    def kernel_cells(self) -> dict:
+        """Cells per launch of k_stress and k_velocity, and of their interior runs."""
+        out = np.zeros(4, dtype=np.int64)
+        self._chk(self.lib.bq_kernel_cells(self.h, _ptr(out)))
+        return dict(zip(("k_stress", "k_velocity", "k_stress_interior", "k_velocity_interior"),
+                        map(int, out)))
 
     def field(self, name: str) -> np.ndarray:
         out = np.zeros((self.nz, self.nx), dtype=np.float32)

@@ -22,7 +22,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -33,8 +35,6 @@ using bq::GridDesc;
 
 namespace {
 
-constexpr int kBX = 64;   // threads per block along x (one wavefront)
-constexpr int kBY = 4;    // rows per block
 constexpr int kEdgeBlock = 256;
 
 struct KParams {
@@ -70,12 +70,10 @@ struct KParams {
 //   l in [MZ', PMLZH-1]  HPLANE_PML_STRESS, LL = l-MZ'+1 (hplane_pml_stress.F90:53-134)
 //   l = PMLZH        HPLANE_PMLZH0_STRESS, TXZ only (hplane_pmlzh0_stress.F90:43-68)
 // and the stress-imaging rows (hplane_0_stress.F90:47-51, hplane_1_stress.F90:44-49).
-__global__ __launch_bounds__(kBX * kBY) void k_stress(KParams k, int itile) {
-  const int ix = blockIdx.x * kBX + threadIdx.x;
-  const int l = 2 + blockIdx.y * kBY + threadIdx.y;
-  if (ix >= k.NX || l > k.PMLZH) return;
+__device__ __forceinline__ void stress_cell(const KParams &k, const int i, const int l,
+                                            const int itile) {
   const int T = k.T, MX = k.MX, MZ = k.MZ, PMLZH = k.PMLZH, NX = k.NX;
-  const int i = ix + k.PMLL;
+  const int ix = i - k.PMLL;
   const bool zh0 = (l == PMLZH);
   const bool pmlrow = (l >= MZ && l < PMLZH);
   const bool zpml = zh0 || pmlrow;
@@ -220,12 +218,9 @@ __global__ __launch_bounds__(kBX * kBY) void k_stress(KParams k, int itile) {
 //                          INH_V_U_2ND (inh_qx_2nd.F90:54-103, inh_v_u_2nd.F90:56-104)
 //                          in the side PML; likewise QZ/W
 //   l in [MZ'-1, PMLZH-1]  HPLANE_PML_VELOCITY, LL = l-MZ'+1 (hplane_pml_velocity.F90:64-180)
-__global__ __launch_bounds__(kBX * kBY) void k_velocity(KParams k) {
-  const int ix = blockIdx.x * kBX + threadIdx.x;
-  const int l = 2 + blockIdx.y * kBY + threadIdx.y;
-  if (ix >= k.NX || l > k.PMLZH - 1) return;
+__device__ __forceinline__ void velocity_cell(const KParams &k, const int i, const int l) {
   const int T = k.T, MX = k.MX, NX = k.NX;
-  const int i = ix + k.PMLL;
+  const int ix = i - k.PMLL;
   const bool pmlrow = (l >= k.MZ - 1);
   const int LL = l - k.MZ + 1;
   const float A = k.A, B = k.B;
@@ -366,6 +361,218 @@ __global__ __launch_bounds__(kBX * kBY) void k_velocity(KParams k) {
   }
 }
 
+
+// ------------------------------------------------------- interior fast path
+// Interior cells (4 <= i <= MX-2, rows of the IC class away from the image
+// rows) use only the 4th-order formulas. A wavefront owns 64 columns of a run
+// of rows; the z-stencil values of its own column are loaded once per run and
+// kept in registers, and the x-neighbours come from the adjacent lanes through
+// cross-lane shuffles, with the 2 columns on either side of the 64 loaded by
+// four halo lanes. The arithmetic is the same expression as the per-cell path.
+
+struct Split {
+  int fx0, ntx, fz0, fz1, ntz, nr;  // interior: ntx 64-column strips from fx0, rows [fz0, fz1),
+                                    // in ntz blocks of 4 runs of nr rows
+  int rx0[4], rx1[4], rz0[4], rz1[4];      // remaining cells: four rectangles [x0,x1) x [z0,z1)
+  int cum[5];                              // cumulative cell counts of the rectangles
+};
+
+// Values of a row at columns i-2, i-1, i+1, i+2 for lane i of a 64-lane tile
+// starting at column c; h holds the halo: lane 0 -> c+64, 1 -> c+65, 2 -> c-1, 3 -> c-2.
+struct XNb {
+  float m2, m1, p1, p2;
+};
+__device__ __forceinline__ XNb xneighbours(float v, float h, int lane) {
+  const float hr0 = __shfl(h, 0), hr1 = __shfl(h, 1), hl1 = __shfl(h, 2), hl2 = __shfl(h, 3);
+  const float sp1 = __shfl(v, (lane + 1) & 63), sp2 = __shfl(v, (lane + 2) & 63);
+  const float sm1 = __shfl(v, (lane + 63) & 63), sm2 = __shfl(v, (lane + 62) & 63);
+  XNb r;
+  r.p1 = lane == 63 ? hr0 : sp1;
+  r.p2 = lane == 63 ? hr1 : (lane == 62 ? hr0 : sp2);
+  r.m1 = lane == 0 ? hl1 : sm1;
+  r.m2 = lane == 0 ? hl2 : (lane == 1 ? hl1 : sm2);
+  return r;
+}
+
+// Short unrolled column runs: a block of 4 wavefronts covers 64 columns x
+// 4*NR rows, wavefront w taking rows l0+w*NR .. l0+w*NR+NR-1 of its lanes'
+// columns. Loads of one run are independent of each other, so all of them can
+// be in flight at once; the z-stencil rows shared by the NR cells of a run are
+// loaded once. Blocks are numbered so that each XCD (block b runs on XCD b%8)
+// gets a contiguous band of tiles, keeping the halo rows shared by vertically
+// adjacent tiles inside one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int per = nblocks >> 3, rem = nblocks & 7, x = b & 7, q = b >> 3;
+  // XCD x owns tiles [start_x, start_x + per + (x < rem)) in row-major order
+  return x * per + min(x, rem) + q;
+}
+
+template <int NR>
+__device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, int tile, int wv,
+                                           int lane, int itile) {
+  const int tx = tile % sp.ntx, tz = tile / sp.ntx;
+  const int c = sp.fx0 + tx * 64;
+  const int i = c + lane;
+  const int l0 = sp.fz0 + (tz * 4 + wv) * NR;
+  if (l0 >= sp.fz1) return;
+  const int P = k.P;
+  const int ix = i + k.OFF;
+  const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
+  const float *__restrict__ U = k.U;
+  const float *__restrict__ W = k.W;
+  const float *__restrict__ QX = k.QX;
+  const float *__restrict__ QZ = k.QZ;
+  const int *__restrict__ JM = k.JM;
+  float *__restrict__ TXX = k.TXX;
+  float *__restrict__ TZZ = k.TZZ;
+  float *__restrict__ TXZ = k.TXZ;
+  float *__restrict__ PR = k.PR;
+  const float AH = k.AH, BH = k.BH, BC = k.BC, C = k.CH, DT = k.DT;
+  // rows l0-2 .. l0+NR+1 of U, W, QZ at this column (z-stencils of the run)
+  // (rows past the last row of a partial run are clamped into the grid; their
+  // values are never used)
+  float u[NR + 3], w[NR + 3], z[NR + 3];
+#pragma unroll
+  for (int r = 0; r < NR + 3; ++r) {
+    const int ru = min(l0 - 2 + r, k.PMLZH), rw = min(l0 - 1 + r, k.PMLZH);
+    u[r] = U[ru * P + ix];
+    w[r] = W[rw * P + ix];   // w[r] = W(l0-1+r)
+    z[r] = QZ[rw * P + ix];
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int l = l0 + j;
+    if (l >= sp.fz1) break;
+    const int r = l * P;
+    const float qx0 = QX[r + ix];
+    const XNb un = xneighbours(u[j + 2], U[r + hx], lane);
+    const XNb wn = xneighbours(w[j + 1], W[r + hx], lane);
+    const XNb qn = xneighbours(qx0, QX[r + hx], lane);
+    const int jm = JM[r + ix] - 1;
+    const float4 s0 = k.stab[3 * jm], s1 = k.stab[3 * jm + 1], s2 = k.stab[3 * jm + 2];
+    const float um2 = u[j], um1 = u[j + 1], u0 = u[j + 2], up1 = u[j + 3];
+    const float wm1 = w[j], w0 = w[j + 1], wp1 = w[j + 2], wp2 = w[j + 3];
+    const float zm1 = z[j], z0 = z[j + 1], zp1 = z[j + 2], zp2 = z[j + 3];
+    const float exz = AH * (up1 - um2 + wn.p1 - wn.m2) + BH * (u0 - um1 + w0 - wn.m1);
+    const float exx = AH * (un.p2 - un.m1) + BH * (un.p1 - u0);
+    const float qxx = AH * (qn.p2 - qn.m1) + BH * (qn.p1 - qx0);
+    float ezz = (C * (wp1 - w0));
+    float qzz = (C * (zp1 - z0));
+    ezz = AH * (wp2 - wm1) + BC * ezz;
+    qzz = AH * (zp2 - zm1) + BC * qzz;
+    TXZ[r + ix] = TXZ[r + ix] + DT * (s0.x * exz);
+    const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
+    const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
+    const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
+    if (i == k.LIN_X && l == k.LIN_Z) {
+      const float2 ds = k.dsrc[itile - 1];
+      TXX[r + ix] = TXX[r + ix] + DT * (rxx - ds.x);
+      TZZ[r + ix] = TZZ[r + ix] + DT * (rzz - ds.x);
+      PR[r + ix] = PR[r + ix] + DT * (rpp + ds.y);
+    } else {
+      TXX[r + ix] = TXX[r + ix] + DT * rxx;
+      TZZ[r + ix] = TZZ[r + ix] + DT * rzz;
+      PR[r + ix] = PR[r + ix] + DT * rpp;
+    }
+  }
+}
+
+template <int NR>
+__device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, int tile, int wv,
+                                             int lane) {
+  const int tx = tile % sp.ntx, tz = tile / sp.ntx;
+  const int c = sp.fx0 + tx * 64;
+  const int i = c + lane;
+  const int l0 = sp.fz0 + (tz * 4 + wv) * NR;
+  if (l0 >= sp.fz1) return;
+  const int P = k.P;
+  const int ix = i + k.OFF;
+  const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
+  const float *__restrict__ TXX = k.TXX;
+  const float *__restrict__ TZZ = k.TZZ;
+  const float *__restrict__ TXZ = k.TXZ;
+  const float *__restrict__ PR = k.PR;
+  const int *__restrict__ JM = k.JM;
+  float *__restrict__ U = k.U;
+  float *__restrict__ W = k.W;
+  float *__restrict__ QX = k.QX;
+  float *__restrict__ QZ = k.QZ;
+  const float A = k.A, B = k.B;
+  // rows l0-2 .. l0+NR+1: xz[r] = TXZ(l0-2+r), zz[r] = TZZ(l0-2+r), pp[r] = PRES(l0-2+r)
+  // (rows past the last row of a partial run are clamped into the grid; their
+  // values are never used)
+  float xz[NR + 4], zz[NR + 3], pp[NR + 3];
+#pragma unroll
+  for (int r = 0; r < NR + 4; ++r) xz[r] = (r == 0) ? 0.0f : TXZ[min(l0 - 2 + r, k.PMLZH) * P + ix];
+#pragma unroll
+  for (int r = 0; r < NR + 3; ++r) {
+    const int rr = min(l0 - 2 + r, k.PMLZH);
+    zz[r] = TZZ[rr * P + ix];
+    pp[r] = PR[rr * P + ix];
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int l = l0 + j;
+    if (l >= sp.fz1) break;
+    const int r = l * P;
+    const float txx0 = TXX[r + ix];
+    const float xz0 = xz[j + 2], p0 = pp[j + 2];
+    const XNb xx = xneighbours(txx0, TXX[r + hx], lane);
+    const XNb xzn = xneighbours(xz0, TXZ[r + hx], lane);
+    const XNb pn = xneighbours(p0, PR[r + hx], lane);
+    const int jm = JM[r + ix] - 1;
+    const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
+    const float q = QX[r + ix], u = U[r + ix], qz = QZ[r + ix], w = W[r + ix];
+    const float sx = A * (xx.p1 - xx.m2 + xz[j + 4] - xz[j + 1]) + B * (txx0 - xx.m1 + xz[j + 3] - xz0);
+    const float px = A * (pn.p1 - pn.m2) + B * (p0 - pn.m1);
+    QX[r + ix] = v0.x * q - v0.y * sx - v0.z * px;
+    U[r + ix] = u + v0.w * q + v1.x * sx + v1.y * px;
+    const float sz = A * (zz[j + 3] - zz[j] + xzn.p2 - xzn.m1) + B * (zz[j + 2] - zz[j + 1] + xzn.p1 - xz0);
+    const float pz = A * (pp[j + 3] - pp[j]) + B * (p0 - pp[j + 1]);
+    QZ[r + ix] = v1.z * qz - v1.w * sz - v2.x * pz;
+    W[r + ix] = w + v2.y * qz + v2.z * sz + v2.w * pz;
+  }
+}
+
+constexpr int kBlock = 256;  // 4 wavefronts
+constexpr int kRun = 4;      // rows per wavefront run
+
+__device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l) {
+  if (g >= sp.cum[4]) return false;
+  int r = 0;
+  while (g >= sp.cum[r + 1]) ++r;
+  const int off = g - sp.cum[r];
+  const int w = sp.rx1[r] - sp.rx0[r];
+  *i = sp.rx0[r] + off % w;
+  *l = sp.rz0[r] + off / w;
+  return true;
+}
+
+// One launch per half-step: the first nrest blocks take the remaining cells
+// one per thread (per-cell path), the others take the interior runs, so the
+// slow boundary cells overlap with the interior instead of trailing it.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_stress(KParams k, Split sp, int itile, int nrest) {
+  if ((int)blockIdx.x < nrest) {
+    int i, l;
+    if (rest_cell(sp, blockIdx.x * kBlock + threadIdx.x, &i, &l)) stress_cell(k, i, l, itile);
+    return;
+  }
+  const int b = xcd_tile(blockIdx.x - nrest, gridDim.x - nrest);
+  stress_run<NR>(k, sp, b, threadIdx.x >> 6, threadIdx.x & 63, itile);
+}
+
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nrest) {
+  if ((int)blockIdx.x < nrest) {
+    int i, l;
+    if (rest_cell(sp, blockIdx.x * kBlock + threadIdx.x, &i, &l)) velocity_cell(k, i, l);
+    return;
+  }
+  const int b = xcd_tile(blockIdx.x - nrest, gridDim.x - nrest);
+  velocity_run<NR>(k, sp, b, threadIdx.x >> 6, threadIdx.x & 63);
+}
+
 // ------------------------------------------------------------------- edges
 // Row 1, stress imaging (HPLANE_1_VELOCITY, hplane_1_velocity.F90:61-76):
 // INH_V_QX_INT_1_SI (inh_v_qx_int_1_si.F90:26-31) for all x first, then
@@ -461,9 +668,19 @@ __global__ void k_sample(KParams k, int itile) {
 // the achievable copy bandwidth of the device.
 __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a,
                                                    float4 *__restrict__ b, size_t n4) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
-       t += (size_t)gridDim.x * blockDim.x)
-    b[t] = a[t];
+  // each block moves 4 x 256 float4 (16 KiB), four independent loads in flight per lane
+  const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const size_t t = base + 256 * u;
+    v[u] = t < n4 ? a[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const size_t t = base + 256 * u;
+    if (t < n4) b[t] = v[u];
+  }
 }
 
 }  // namespace
@@ -487,10 +704,45 @@ struct bq_handle {
   std::vector<hipEvent_t> ev;
   double ktime[4] = {0, 0, 0, 0};
   int timed_steps = 0;
+  Split ss{}, sv{};
   std::string err;
 };
 
 namespace {
+
+// Partition of the cells a half-step updates, x in [PMLL, PMLXH] and rows
+// [z_lo, z_hi], into 64-column fast tiles over [fx_lo, fx_hi] x [fz0, fz1)
+// and four rectangles of remaining cells. The fast paths read rows up to
+// fz1+2 (z-stencil plus the one-row load-ahead) and columns fx_lo-2 ..
+// fx_hi+2; with fz1 <= MZ'-1, TPML >= 1 and 4 <= fx_lo, fx_hi <= MX-2 these
+// stay inside the grid.
+Split make_split(const GridDesc &g, int z_lo, int z_hi, int fx_lo, int fx_hi, int fz0, int fz1,
+                 int nr) {
+  Split s{};
+  s.nr = nr;
+  s.fx0 = fx_lo;
+  s.ntx = fx_hi >= fx_lo ? (fx_hi - fx_lo + 1) / 64 : 0;
+  s.fz0 = fz0;
+  s.fz1 = fz1;
+  s.ntz = (fz1 > fz0 && s.ntx > 0) ? (fz1 - fz0 + 4 * nr - 1) / (4 * nr) : 0;
+  if (s.ntz == 0) s.ntx = 0;
+  const int fxe = s.fx0 + 64 * s.ntx;  // first column after the fast tiles
+  if (s.ntx * s.ntz == 0) {  // everything on the per-cell path
+    s.rx0[0] = g.PMLL; s.rx1[0] = g.PMLXH + 1; s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
+    for (int r = 1; r < 4; ++r) { s.rx0[r] = s.rx1[r] = 0; s.rz0[r] = s.rz1[r] = 0; }
+  } else {
+    s.rx0[0] = g.PMLL; s.rx1[0] = s.fx0;       s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
+    s.rx0[1] = fxe;    s.rx1[1] = g.PMLXH + 1; s.rz0[1] = z_lo; s.rz1[1] = z_hi + 1;
+    s.rx0[2] = s.fx0;  s.rx1[2] = fxe;         s.rz0[2] = z_lo; s.rz1[2] = fz0;
+    s.rx0[3] = s.fx0;  s.rx1[3] = fxe;         s.rz0[3] = fz1;  s.rz1[3] = z_hi + 1;
+  }
+  s.cum[0] = 0;
+  for (int r = 0; r < 4; ++r) {
+    const int w = s.rx1[r] - s.rx0[r], hgt = s.rz1[r] - s.rz0[r];
+    s.cum[r + 1] = s.cum[r] + (w > 0 && hgt > 0 ? w * hgt : 0);
+  }
+  return s;
+}
 
 int fail(bq_handle *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -609,6 +861,14 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   k.dsrc = ds;
   if (rc) { int r = rc; bq_destroy(h); return r < 0 ? r : -101; }
   if (hipStreamSynchronize(h->stream) != hipSuccess) { bq_destroy(h); return -100; }
+  // interior tiles start at the first column >= 4 whose address is 128-byte
+  // aligned, so a wavefront's 64 values of a row are exactly two cache lines
+  int fx = 4;
+  while ((fx + k.OFF) % 32 != 0) ++fx;
+  // stress: rows 2..PMLZH; interior rows 5..MZ'-2 (rows 2-4 write the image rows)
+  h->ss = make_split(g, 2, g.PMLZH, fx, g.MX - 2, 5, g.MZ - 1, kRun);
+  // velocity: rows 2..PMLZH-1; interior rows 2..MZ'-2
+  h->sv = make_split(g, 2, g.PMLZH - 1, fx, g.MX - 2, 2, g.MZ - 1, kRun);
   *out = h;
   return 0;
 }
@@ -802,13 +1062,14 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   if (itile_end < itile_begin) return 0;
   const KParams &k = h->kp;
   const GridDesc &g = h->g;
-  const dim3 blk(kBX, kBY);
-  const dim3 gs((g.NX + kBX - 1) / kBX, (g.PMLZH - 1 + kBY - 1) / kBY);  // rows 2..PMLZH
-  const dim3 gv((g.NX + kBX - 1) / kBX, (g.PMLZH - 2 + kBY - 1) / kBY);  // rows 2..PMLZH-1
+  const dim3 blk(kBlock);
+  const int srest = (h->ss.cum[4] + kBlock - 1) / kBlock, vrest = (h->sv.cum[4] + kBlock - 1) / kBlock;
+  const int sgrid = srest + h->ss.ntx * h->ss.ntz, vgrid = vrest + h->sv.ntx * h->sv.ntz;
   const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
   const int nsteps = itile_end - itile_begin + 1;
+  constexpr int kEv = 5;
   if (h->timing) {
-    const size_t need = (size_t)nsteps * 5;
+    const size_t need = (size_t)nsteps * kEv;
     while (h->ev.size() < need) {
       hipEvent_t e;
       HIPCHK(h, hipEventCreate(&e));
@@ -816,11 +1077,11 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     }
   }
   for (int it = itile_begin; it <= itile_end; ++it) {
-    hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * 5] : nullptr;
+    hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * kEv] : nullptr;
     if (e) (void)hipEventRecord(e[0], h->stream);
-    hipLaunchKernelGGL(k_stress, gs, blk, 0, h->stream, k, it);
+    if (sgrid) hipLaunchKernelGGL(k_stress<kRun>, dim3(sgrid), blk, 0, h->stream, k, h->ss, it, srest);
     if (e) (void)hipEventRecord(e[1], h->stream);
-    hipLaunchKernelGGL(k_velocity, gv, blk, 0, h->stream, k);
+    if (vgrid) hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
     if (e) (void)hipEventRecord(e[2], h->stream);
     hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
     if (e) (void)hipEventRecord(e[3], h->stream);
@@ -831,17 +1092,26 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   h->next_step = itile_end + 1;
   if (h->timing) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    double acc[4] = {0, 0, 0, 0};
+    double acc[kEv - 1] = {0, 0, 0, 0};
     for (int s = 0; s < nsteps; ++s) {
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kEv - 1; ++q) {
         float ms = 0;
-        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[(size_t)s * 5 + q], h->ev[(size_t)s * 5 + q + 1]));
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[(size_t)s * kEv + q], h->ev[(size_t)s * kEv + q + 1]));
         acc[q] += ms;
       }
     }
-    for (int q = 0; q < 4; ++q) h->ktime[q] = acc[q] / nsteps;
+    for (int q = 0; q < kEv - 1; ++q) h->ktime[q] = acc[q] / nsteps;
     h->timed_steps = nsteps;
   }
+  return 0;
+}
+
+int bq_kernel_cells(bq_handle *h, long long out[4]) {
+  if (!h || !out) return -1;
+  out[0] = 64LL * h->ss.ntx * (h->ss.fz1 - h->ss.fz0) + h->ss.cum[4];
+  out[1] = 64LL * h->sv.ntx * (h->sv.fz1 - h->sv.fz0) + h->sv.cum[4];
+  out[2] = 64LL * h->ss.ntx * (h->ss.fz1 - h->ss.fz0);
+  out[3] = 64LL * h->sv.ntx * (h->sv.fz1 - h->sv.fz0);
   return 0;
 }
 
@@ -862,7 +1132,7 @@ int bq_probe_copy(long long nfloats, int iters, double *gbps) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const int blocks = 256 * 16;
+  const int blocks = (int)((n4 + 1023) / 1024);
   hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);
   (void)hipEventRecord(e0, 0);
   for (int r = 0; r < iters; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);

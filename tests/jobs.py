@@ -37,3 +37,21 @@ def tiny_edges(steps=150, tpml=5, mx=36, mz=24, f0=90.0, seed=7) -> Job:
 
 
 ALL = {"layered": tiny_layered, "homog": tiny_homog, "edges": tiny_edges}
+
+
+def medium_layered(steps=200, tpml=10, mx=200, mz=120, seed=5, f0=40.0) -> Job:
+    """Large enough that the interior fast tiles (64 columns wide) are used."""
+    return models.layered_job(name="MEDIUM", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=4, seed=seed,
+                              tpml=tpml, f0=f0, n_rec=40)
+
+
+def medium_deep(steps=200, tpml=12, mx=230, mz=140, f0=45.0) -> Job:
+    """Correlated random model, source deep inside a fast tile."""
+    job = models.layered_job(name="DEEP", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=3, seed=9,
+                             tpml=tpml, f0=f0, n_rec=30, field_std=0.08, corr_len=20.0, levels=6)
+    job.x_source = 151.0 * 2.0
+    job.z_source = 70.0 * 2.0
+    return job
+
+
+GPU_EXTRA = {"medium": medium_layered, "deep": medium_deep}

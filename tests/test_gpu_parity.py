@@ -23,9 +23,12 @@ def _compare(sim, ref, where):
         assert bad.size == 0, f"{where}: field {f} differs at {len(bad)} cells, first {bad[0]}: {a[tuple(bad[0])]} vs {b[tuple(bad[0])]}"
 
 
-@pytest.mark.parametrize("name", sorted(jobs.ALL))
+CASES = {**jobs.ALL, **jobs.GPU_EXTRA}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
 def test_bitexact_vs_oracle(name):
-    job = jobs.ALL[name]()
+    job = CASES[name]()
     print(f"case {name}: grid {job.mx}x{job.mz} T={job.tpml} steps={job.mt2}", flush=True)
     ref = oracle.OracleSim(job, "f32")
     sim = lib.Simulation(job)
