@@ -40,7 +40,10 @@ constexpr int kEdgeBlock = 256;
 struct KParams {
   int MX, MZ, T, PMLL, PMLZH, NX, P, OFF, LIN_X, LIN_Z, MR;
   float DT, A, B, AH, BH, CH, BC;
-  float *U, *W, *QX, *QZ, *TXX, *TZZ, *TXZ, *PR;
+  // fields of the current step (read) and of the next step (written); the two
+  // sets trade places after every step
+  const float *U0, *W0, *QX0, *QZ0, *TXX0, *TZZ0, *TXZ0, *PR0;
+  float *U1, *W1, *QX1, *QZ1, *TXX1, *TZZ1, *TXZ1, *PR1;
   const int *JM;
   const float4 *stab;  // per material: {MU,SXX1,SXX2,SXX3},{SZZ1,SZZ2,SZZ3,P1},{P2,P3,-,-}
   const float4 *vtab;  // per material: {eQX,R1QX,R2QX,fU},{R1U,R3U,eQZ,R1QZ},{R2QZ,fW,R1W,R3W}
@@ -84,29 +87,29 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
 
   // ---- shear strain and TXZ: STRAIN_UW_XZ (strain_uw_xz.F90:61-103) or
   //      STRAIN_UW_XZ_2ND_PML (strain_uw_xz_2nd_pml.F90:69-121), STRESS_XZ
-  float txz = FA(k.TXZ, i, l);
+  float txz = FA(k.TXZ0, i, l);
   if (i >= 2 - T && i <= MX + T) {
     float exz;
     if (!zpml) {
       if (i <= 2) {
-        const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
+        const float dzxa = C * (FA(k.W0, i, l) - FA(k.W0, i - 1, l));
         float &pm = k.PZXL[(size_t)l * (T + 2) + ix];
         const int j = 3 - i;
-        exz = C * (FA(k.U, i, l) - FA(k.U, i, l - 1)) + XL2(0, j, l) * (XL2(3, j, l) * dzxa + pm);
+        exz = C * (FA(k.U0, i, l) - FA(k.U0, i, l - 1)) + XL2(0, j, l) * (XL2(3, j, l) * dzxa + pm);
         pm = XL2(1, j, l) * pm + XL2(2, j, l) * dzxa;
       } else if (i <= MX - 1) {
-        exz = k.AH * (FA(k.U, i, l + 1) - FA(k.U, i, l - 2) + FA(k.W, i + 1, l) - FA(k.W, i - 2, l)) +
-              k.BH * (FA(k.U, i, l) - FA(k.U, i, l - 1) + FA(k.W, i, l) - FA(k.W, i - 1, l));
+        exz = k.AH * (FA(k.U0, i, l + 1) - FA(k.U0, i, l - 2) + FA(k.W0, i + 1, l) - FA(k.W0, i - 2, l)) +
+              k.BH * (FA(k.U0, i, l) - FA(k.U0, i, l - 1) + FA(k.W0, i, l) - FA(k.W0, i - 1, l));
       } else {
-        const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
+        const float dzxa = C * (FA(k.W0, i, l) - FA(k.W0, i - 1, l));
         float &pm = k.PZXH[(size_t)l * (T + 1) + (i - MX)];
         const int j = i - (MX - 1);
-        exz = C * (FA(k.U, i, l) - FA(k.U, i, l - 1)) + XL2(4, j, l) * (XL2(7, j, l) * dzxa + pm);
+        exz = C * (FA(k.U0, i, l) - FA(k.U0, i, l - 1)) + XL2(4, j, l) * (XL2(7, j, l) * dzxa + pm);
         pm = XL2(5, j, l) * pm + XL2(6, j, l) * dzxa;
       }
     } else {
-      const float dzxa = C * (FA(k.W, i, l) - FA(k.W, i - 1, l));
-      const float dxza = C * (FA(k.U, i, l) - FA(k.U, i, l - 1));
+      const float dzxa = C * (FA(k.W0, i, l) - FA(k.W0, i - 1, l));
+      const float dxza = C * (FA(k.U0, i, l) - FA(k.U0, i, l - 1));
       float &pz = k.PXZH[(size_t)LL * NX + ix];
       const float zp = ZH2(0, LL, ix) * (ZH2(3, LL, ix) * dxza + pz);
       if (i <= 2) {
@@ -127,41 +130,41 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
     txz = txz + DT * (s0.x * exz);
   }
   // TXZ(:,2) = 0 after the images are taken (hplane_1_stress.F90:48)
-  FA(k.TXZ, i, l) = (l == 2) ? 0.0f : txz;
-  if (l == 3) FA(k.TXZ, i, 1) = -txz;
-  if (l == 4) FA(k.TXZ, i, 0) = -txz;
+  FA(k.TXZ1, i, l) = (l == 2) ? 0.0f : txz;
+  if (l == 3) FA(k.TXZ1, i, 1) = -txz;
+  if (l == 4) FA(k.TXZ1, i, 0) = -txz;
   if (zh0) return;
 
   // ---- normal strains and TXX, TZZ, PRES: STRAIN_UW_XX / STRAIN_Q_XX
   //      (strain_uw_xx.F90:60-107), STRAIN_UW_ZZ / STRAIN_Q_ZZ (strain_uw_zz.F90:49-60)
   //      or their _2ND_PML forms (strain_uw_zz_2nd_pml.F90:55-65), STRESS_NN /
   //      STRESS_LIN_NN (stress_nn.F90:49-61, stress_lin_nn.F90:49-70)
-  float tzz = FA(k.TZZ, i, l), pr = FA(k.PR, i, l);
+  float tzz = FA(k.TZZ0, i, l), pr = FA(k.PR0, i, l);
   if (i >= 2 - T && i <= MX - 1 + T) {
     float exx, qxx, ezz, qzz;
     if (i <= 1) {
       const int j = 2 - i;
       const size_t mo = (size_t)l * (T + 1) + ix;
       const float a = XL1(0, j, l), b = XL1(1, j, l), c = XL1(2, j, l), d = XL1(3, j, l);
-      const float du = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
-      const float dq = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      const float du = C * (FA(k.U0, i + 1, l) - FA(k.U0, i, l));
+      const float dq = C * (FA(k.QX0, i + 1, l) - FA(k.QX0, i, l));
       float pu = k.PXXL[mo], pq = k.OXXL[mo];
       exx = a * (d * du + pu);
       qxx = a * (d * dq + pq);
       k.PXXL[mo] = b * pu + c * du;
       k.OXXL[mo] = b * pq + c * dq;
     } else if (i == 2 || i == MX - 1) {
-      exx = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
-      qxx = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      exx = C * (FA(k.U0, i + 1, l) - FA(k.U0, i, l));
+      qxx = C * (FA(k.QX0, i + 1, l) - FA(k.QX0, i, l));
     } else if (i <= MX - 2) {
-      exx = k.AH * (FA(k.U, i + 2, l) - FA(k.U, i - 1, l)) + k.BH * (FA(k.U, i + 1, l) - FA(k.U, i, l));
-      qxx = k.AH * (FA(k.QX, i + 2, l) - FA(k.QX, i - 1, l)) + k.BH * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      exx = k.AH * (FA(k.U0, i + 2, l) - FA(k.U0, i - 1, l)) + k.BH * (FA(k.U0, i + 1, l) - FA(k.U0, i, l));
+      qxx = k.AH * (FA(k.QX0, i + 2, l) - FA(k.QX0, i - 1, l)) + k.BH * (FA(k.QX0, i + 1, l) - FA(k.QX0, i, l));
     } else {
       const int j = i - (MX - 1);
       const size_t mo = (size_t)l * (T + 1) + (i - MX);
       const float a = XL1(4, j, l), b = XL1(5, j, l), c = XL1(6, j, l), d = XL1(7, j, l);
-      const float du = C * (FA(k.U, i + 1, l) - FA(k.U, i, l));
-      const float dq = C * (FA(k.QX, i + 1, l) - FA(k.QX, i, l));
+      const float du = C * (FA(k.U0, i + 1, l) - FA(k.U0, i, l));
+      const float dq = C * (FA(k.QX0, i + 1, l) - FA(k.QX0, i, l));
       float pu = k.PXXH[mo], pq = k.OXXH[mo];
       exx = a * (d * du + pu);
       qxx = a * (d * dq + pq);
@@ -169,16 +172,16 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
       k.OXXH[mo] = b * pq + c * dq;
     }
     if (!pmlrow && l != MZ - 1) {
-      ezz = (C * (FA(k.W, i, l + 1) - FA(k.W, i, l)));
-      qzz = (C * (FA(k.QZ, i, l + 1) - FA(k.QZ, i, l)));
+      ezz = (C * (FA(k.W0, i, l + 1) - FA(k.W0, i, l)));
+      qzz = (C * (FA(k.QZ0, i, l + 1) - FA(k.QZ0, i, l)));
       if (i >= 2 && i <= MX - 1) {
-        ezz = k.AH * (FA(k.W, i, l + 2) - FA(k.W, i, l - 1)) + k.BC * ezz;
-        qzz = k.AH * (FA(k.QZ, i, l + 2) - FA(k.QZ, i, l - 1)) + k.BC * qzz;
+        ezz = k.AH * (FA(k.W0, i, l + 2) - FA(k.W0, i, l - 1)) + k.BC * ezz;
+        qzz = k.AH * (FA(k.QZ0, i, l + 2) - FA(k.QZ0, i, l - 1)) + k.BC * qzz;
       }
     } else {
       const float a = ZH1(0, LL, ix), b = ZH1(1, LL, ix), c = ZH1(2, LL, ix), d = ZH1(3, LL, ix);
-      const float dw = C * (FA(k.W, i, l + 1) - FA(k.W, i, l));
-      const float dq = C * (FA(k.QZ, i, l + 1) - FA(k.QZ, i, l));
+      const float dw = C * (FA(k.W0, i, l + 1) - FA(k.W0, i, l));
+      const float dq = C * (FA(k.QZ0, i, l + 1) - FA(k.QZ0, i, l));
       const size_t mo = (size_t)LL * NX + ix;
       float pw = k.PZZH[mo], pq = k.OZZH[mo];
       ezz = a * (d * dw + pw);
@@ -191,7 +194,7 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
     const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
     const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
     const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
-    float txx = FA(k.TXX, i, l);
+    float txx = FA(k.TXX0, i, l);
     if (l == k.LIN_Z && i == k.LIN_X) {
       const float2 ds = k.dsrc[itile - 1];
       txx = txx + DT * (rxx - ds.x);
@@ -202,12 +205,12 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
       tzz = tzz + DT * rzz;
       pr = pr + DT * rpp;
     }
-    FA(k.TXX, i, l) = txx;
-    FA(k.TZZ, i, l) = tzz;
-    FA(k.PR, i, l) = pr;
+    FA(k.TXX1, i, l) = txx;
+    FA(k.TZZ1, i, l) = tzz;
+    FA(k.PR1, i, l) = pr;
   }
-  if (l == 2) { FA(k.TZZ, i, 1) = -tzz; FA(k.PR, i, 1) = -pr; }
-  if (l == 3) { FA(k.TZZ, i, 0) = -tzz; FA(k.PR, i, 0) = -pr; }
+  if (l == 2) { FA(k.TZZ1, i, 1) = -tzz; FA(k.PR1, i, 1) = -pr; }
+  if (l == 3) { FA(k.TZZ1, i, 0) = -tzz; FA(k.PR1, i, 0) = -pr; }
 }
 
 // ---------------------------------------------------------------- velocity
@@ -230,15 +233,15 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
 
   // ---- QX and U, x in [3-T, MX-1+T]
   if (i >= 3 - T && i <= MX - 1 + T) {
-    const float q = FA(k.QX, i, l);
-    const float u = FA(k.U, i, l);
+    const float q = FA(k.QX0, i, l);
+    const float u = FA(k.U0, i, l);
     float qn, un;
     const bool inner = (i >= 4 && i <= MX - 2);
     if (!pmlrow) {
       if (inner) {
-        const float sx = A * (FA(k.TXX, i + 1, l) - FA(k.TXX, i - 2, l) + FA(k.TXZ, i, l + 2) - FA(k.TXZ, i, l - 1)) +
-                         B * (FA(k.TXX, i, l) - FA(k.TXX, i - 1, l) + FA(k.TXZ, i, l + 1) - FA(k.TXZ, i, l));
-        const float px = A * (FA(k.PR, i + 1, l) - FA(k.PR, i - 2, l)) + B * (FA(k.PR, i, l) - FA(k.PR, i - 1, l));
+        const float sx = A * (FA(k.TXX1, i + 1, l) - FA(k.TXX1, i - 2, l) + FA(k.TXZ1, i, l + 2) - FA(k.TXZ1, i, l - 1)) +
+                         B * (FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l) + FA(k.TXZ1, i, l + 1) - FA(k.TXZ1, i, l));
+        const float px = A * (FA(k.PR1, i + 1, l) - FA(k.PR1, i - 2, l)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i - 1, l));
         qn = v0.x * q - v0.y * sx - v0.z * px;
         un = u + v0.w * q + v1.x * sx + v1.y * px;
       } else {
@@ -250,9 +253,9 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         float *G = right ? k.GXXH : k.GXXL, *H = right ? k.HXXH : k.HXXL;
         float *R = right ? k.RXXH : k.RXXL, *S = right ? k.SXXH : k.SXXL;
         const float g = G[mo], hh = H[mo], r = R[mo], s = S[mo];
-        const float dxx = FA(k.TXX, i, l) - FA(k.TXX, i - 1, l);
-        const float pxx = FA(k.PR, i, l) - FA(k.PR, i - 1, l);
-        const float tz1 = FA(k.TXZ, i, l + 1), tz0 = FA(k.TXZ, i, l);
+        const float dxx = FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l);
+        const float pxx = FA(k.PR1, i, l) - FA(k.PR1, i - 1, l);
+        const float tz1 = FA(k.TXZ1, i, l + 1), tz0 = FA(k.TXZ1, i, l);
         qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1 - tz0) - v0.z * (oa * (od * pxx + hh));
         un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1 - tz0) + v1.y * (oa * (od * pxx + s));
         G[mo] = ob * g + oc * dxx;
@@ -263,9 +266,9 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
     } else {
       // INH_QX_INT_2ND_PML (inh_qx_int_2nd_pml.F90:60-143), INH_V_U_INT_2ND_PML
       // (inh_v_u_int_2nd_pml.F90:62-143)
-      const float dxx = FA(k.TXX, i, l) - FA(k.TXX, i - 1, l);
-      const float dxz = FA(k.TXZ, i, l + 1) - FA(k.TXZ, i, l);
-      const float pxx = FA(k.PR, i, l) - FA(k.PR, i - 1, l);
+      const float dxx = FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l);
+      const float dxz = FA(k.TXZ1, i, l + 1) - FA(k.TXZ1, i, l);
+      const float pxx = FA(k.PR1, i, l) - FA(k.PR1, i - 1, l);
       const float za = ZH1(0, LL, ix), zb = ZH1(1, LL, ix), zc = ZH1(2, LL, ix), zd = ZH1(3, LL, ix);
       const size_t zo = (size_t)LL * NX + ix;
       const float gz = k.GXZH[zo], rz = k.RXZH[zo];
@@ -291,22 +294,22 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
       k.GXZH[zo] = zb * gz + zc * dxz;
       k.RXZH[zo] = zb * rz + zc * dxz;
     }
-    FA(k.QX, i, l) = qn;
-    FA(k.U, i, l) = un;
+    FA(k.QX1, i, l) = qn;
+    FA(k.U1, i, l) = un;
   }
 
   // ---- QZ and W, x in [2-T, MX-1+T]
   if (i >= 2 - T && i <= MX - 1 + T) {
-    const float q = FA(k.QZ, i, l);
-    const float w = FA(k.W, i, l);
+    const float q = FA(k.QZ0, i, l);
+    const float w = FA(k.W0, i, l);
     float qn, wn;
     const bool inner = (i >= 3 && i <= MX - 2);
     if (!pmlrow) {
       if (inner) {
         // INH_QZ_INT (inh_qz_int.F90:41-55), INH_V_W_INT (inh_v_w_int.F90:43-58)
-        const float sz = A * (FA(k.TZZ, i, l + 1) - FA(k.TZZ, i, l - 2) + FA(k.TXZ, i + 2, l) - FA(k.TXZ, i - 1, l)) +
-                         B * (FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1) + FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l));
-        const float pz = A * (FA(k.PR, i, l + 1) - FA(k.PR, i, l - 2)) + B * (FA(k.PR, i, l) - FA(k.PR, i, l - 1));
+        const float sz = A * (FA(k.TZZ1, i, l + 1) - FA(k.TZZ1, i, l - 2) + FA(k.TXZ1, i + 2, l) - FA(k.TXZ1, i - 1, l)) +
+                         B * (FA(k.TZZ1, i, l) - FA(k.TZZ1, i, l - 1) + FA(k.TXZ1, i + 1, l) - FA(k.TXZ1, i, l));
+        const float pz = A * (FA(k.PR1, i, l + 1) - FA(k.PR1, i, l - 2)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i, l - 1));
         qn = v1.z * q - v1.w * sz - v2.x * pz;
         wn = w + v2.y * q + v2.z * sz + v2.w * pz;
       } else {
@@ -318,9 +321,9 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         const size_t mo = right ? (size_t)l * (T + 2) + (i - (MX - 1)) : (size_t)l * (T + 2) + ix;
         float *G = right ? k.GZXH : k.GZXL, *R = right ? k.RZXH : k.RZXL;
         const float g = G[mo], r = R[mo];
-        const float dxz = FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l);
-        const float dzz = FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1);
-        const float pzz = FA(k.PR, i, l) - FA(k.PR, i, l - 1);
+        const float dxz = FA(k.TXZ1, i + 1, l) - FA(k.TXZ1, i, l);
+        const float dzz = FA(k.TZZ1, i, l) - FA(k.TZZ1, i, l - 1);
+        const float pzz = FA(k.PR1, i, l) - FA(k.PR1, i, l - 1);
         qn = v1.z * q - v1.w * (dzz + oa * (od * dxz + g)) - v2.x * pzz;
         wn = w + v2.y * q + v2.z * (dzz + oa * (od * dxz + r)) + v2.w * pzz;
         G[mo] = ob * g + oc * dxz;
@@ -329,9 +332,9 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
     } else {
       // INH_QZ_INT_2ND_PML (inh_qz_int_2nd_pml.F90:57-144), INH_V_W_INT_2ND_PML
       // (inh_v_w_int_2nd_pml.F90:61-146)
-      const float dxz = FA(k.TXZ, i + 1, l) - FA(k.TXZ, i, l);
-      const float dzz = FA(k.TZZ, i, l) - FA(k.TZZ, i, l - 1);
-      const float pzz = FA(k.PR, i, l) - FA(k.PR, i, l - 1);
+      const float dxz = FA(k.TXZ1, i + 1, l) - FA(k.TXZ1, i, l);
+      const float dzz = FA(k.TZZ1, i, l) - FA(k.TZZ1, i, l - 1);
+      const float pzz = FA(k.PR1, i, l) - FA(k.PR1, i, l - 1);
       const float za = ZH2(0, LL, ix), zb = ZH2(1, LL, ix), zc = ZH2(2, LL, ix), zd = ZH2(3, LL, ix);
       const size_t zo = (size_t)LL * NX + ix;
       const float gz = k.GZZH[zo], hz = k.HZZH[zo], rz = k.RZZH[zo], sz = k.SZZH[zo];
@@ -356,8 +359,8 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
       k.RZZH[zo] = zb * rz + zc * dzz;
       k.SZZH[zo] = zb * sz + zc * pzz;
     }
-    FA(k.QZ, i, l) = qn;
-    FA(k.W, i, l) = wn;
+    FA(k.QZ1, i, l) = qn;
+    FA(k.W1, i, l) = wn;
   }
 }
 
@@ -418,15 +421,19 @@ __device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, in
   const int P = k.P;
   const int ix = i + k.OFF;
   const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
-  const float *__restrict__ U = k.U;
-  const float *__restrict__ W = k.W;
-  const float *__restrict__ QX = k.QX;
-  const float *__restrict__ QZ = k.QZ;
+  const float *__restrict__ U = k.U0;
+  const float *__restrict__ W = k.W0;
+  const float *__restrict__ QX = k.QX0;
+  const float *__restrict__ QZ = k.QZ0;
   const int *__restrict__ JM = k.JM;
-  float *__restrict__ TXX = k.TXX;
-  float *__restrict__ TZZ = k.TZZ;
-  float *__restrict__ TXZ = k.TXZ;
-  float *__restrict__ PR = k.PR;
+  const float *__restrict__ TXX = k.TXX0;
+  const float *__restrict__ TZZ = k.TZZ0;
+  const float *__restrict__ TXZ = k.TXZ0;
+  const float *__restrict__ PR = k.PR0;
+  float *__restrict__ nTXX = k.TXX1;
+  float *__restrict__ nTZZ = k.TZZ1;
+  float *__restrict__ nTXZ = k.TXZ1;
+  float *__restrict__ nPR = k.PR1;
   const float AH = k.AH, BH = k.BH, BC = k.BC, C = k.CH, DT = k.DT;
   // rows l0-2 .. l0+NR+1 of U, W, QZ at this column (z-stencils of the run)
   // (rows past the last row of a partial run are clamped into the grid; their
@@ -460,19 +467,19 @@ __device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, in
     float qzz = (C * (zp1 - z0));
     ezz = AH * (wp2 - wm1) + BC * ezz;
     qzz = AH * (zp2 - zm1) + BC * qzz;
-    TXZ[r + ix] = TXZ[r + ix] + DT * (s0.x * exz);
+    nTXZ[r + ix] = TXZ[r + ix] + DT * (s0.x * exz);
     const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
     const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
     const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
     if (i == k.LIN_X && l == k.LIN_Z) {
       const float2 ds = k.dsrc[itile - 1];
-      TXX[r + ix] = TXX[r + ix] + DT * (rxx - ds.x);
-      TZZ[r + ix] = TZZ[r + ix] + DT * (rzz - ds.x);
-      PR[r + ix] = PR[r + ix] + DT * (rpp + ds.y);
+      nTXX[r + ix] = TXX[r + ix] + DT * (rxx - ds.x);
+      nTZZ[r + ix] = TZZ[r + ix] + DT * (rzz - ds.x);
+      nPR[r + ix] = PR[r + ix] + DT * (rpp + ds.y);
     } else {
-      TXX[r + ix] = TXX[r + ix] + DT * rxx;
-      TZZ[r + ix] = TZZ[r + ix] + DT * rzz;
-      PR[r + ix] = PR[r + ix] + DT * rpp;
+      nTXX[r + ix] = TXX[r + ix] + DT * rxx;
+      nTZZ[r + ix] = TZZ[r + ix] + DT * rzz;
+      nPR[r + ix] = PR[r + ix] + DT * rpp;
     }
   }
 }
@@ -488,15 +495,19 @@ __device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, 
   const int P = k.P;
   const int ix = i + k.OFF;
   const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
-  const float *__restrict__ TXX = k.TXX;
-  const float *__restrict__ TZZ = k.TZZ;
-  const float *__restrict__ TXZ = k.TXZ;
-  const float *__restrict__ PR = k.PR;
+  const float *__restrict__ TXX = k.TXX1;
+  const float *__restrict__ TZZ = k.TZZ1;
+  const float *__restrict__ TXZ = k.TXZ1;
+  const float *__restrict__ PR = k.PR1;
   const int *__restrict__ JM = k.JM;
-  float *__restrict__ U = k.U;
-  float *__restrict__ W = k.W;
-  float *__restrict__ QX = k.QX;
-  float *__restrict__ QZ = k.QZ;
+  const float *__restrict__ U = k.U0;
+  const float *__restrict__ W = k.W0;
+  const float *__restrict__ QX = k.QX0;
+  const float *__restrict__ QZ = k.QZ0;
+  float *__restrict__ nU = k.U1;
+  float *__restrict__ nW = k.W1;
+  float *__restrict__ nQX = k.QX1;
+  float *__restrict__ nQZ = k.QZ1;
   const float A = k.A, B = k.B;
   // rows l0-2 .. l0+NR+1: xz[r] = TXZ(l0-2+r), zz[r] = TZZ(l0-2+r), pp[r] = PRES(l0-2+r)
   // (rows past the last row of a partial run are clamped into the grid; their
@@ -525,12 +536,12 @@ __device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, 
     const float q = QX[r + ix], u = U[r + ix], qz = QZ[r + ix], w = W[r + ix];
     const float sx = A * (xx.p1 - xx.m2 + xz[j + 4] - xz[j + 1]) + B * (txx0 - xx.m1 + xz[j + 3] - xz0);
     const float px = A * (pn.p1 - pn.m2) + B * (p0 - pn.m1);
-    QX[r + ix] = v0.x * q - v0.y * sx - v0.z * px;
-    U[r + ix] = u + v0.w * q + v1.x * sx + v1.y * px;
+    nQX[r + ix] = v0.x * q - v0.y * sx - v0.z * px;
+    nU[r + ix] = u + v0.w * q + v1.x * sx + v1.y * px;
     const float sz = A * (zz[j + 3] - zz[j] + xzn.p2 - xzn.m1) + B * (zz[j + 2] - zz[j + 1] + xzn.p1 - xz0);
     const float pz = A * (pp[j + 3] - pp[j]) + B * (p0 - pp[j + 1]);
-    QZ[r + ix] = v1.z * qz - v1.w * sz - v2.x * pz;
-    W[r + ix] = w + v2.y * qz + v2.z * sz + v2.w * pz;
+    nQZ[r + ix] = v1.z * qz - v1.w * sz - v2.x * pz;
+    nW[r + ix] = w + v2.y * qz + v2.z * sz + v2.w * pz;
   }
 }
 
@@ -591,8 +602,8 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
     const int jx = ix0 + t;
     const int i = jx + k.PMLL;
     if (jx < NX && i >= 3 - T && i <= MX - 1 + T) {
-      const float u1 = FA(k.U, i, 2) + FA(k.W, i, 2) - FA(k.W, i - 1, 2);
-      const float q1 = FA(k.QX, i, 2) + FA(k.QZ, i, 2) - FA(k.QZ, i - 1, 2);
+      const float u1 = FA(k.U1, i, 2) + FA(k.W1, i, 2) - FA(k.W1, i - 1, 2);
+      const float q1 = FA(k.QX1, i, 2) + FA(k.QZ1, i, 2) - FA(k.QZ1, i - 1, 2);
       su[t] = u1;
       sq[t] = q1;
     }
@@ -601,8 +612,8 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
   if (ix < NX) {
     const int i = ix + k.PMLL;
     if (i >= 3 - T && i <= MX - 1 + T) {
-      FA(k.U, i, 1) = su[threadIdx.x];
-      FA(k.QX, i, 1) = sq[threadIdx.x];
+      FA(k.U1, i, 1) = su[threadIdx.x];
+      FA(k.QX1, i, 1) = sq[threadIdx.x];
     }
     // phase 2
     if (i >= 3 - T && i <= MX - 2 + T) {
@@ -610,14 +621,14 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
       const float2 aux = k.atab[jm];
       const float u0 = su[threadIdx.x], u1 = su[threadIdx.x + 1];
       const float q0 = sq[threadIdx.x], q1 = sq[threadIdx.x + 1];
-      const float du = u1 - u0 + FA(k.U, i + 1, 2) - FA(k.U, i, 2);
-      FA(k.W, i, 1) = FA(k.W, i, 3) + aux.x * du;
-      FA(k.QZ, i, 1) = FA(k.QZ, i, 3) + aux.y * du + (q1 - q0 + FA(k.QX, i + 1, 2) - FA(k.QX, i, 2));
+      const float du = u1 - u0 + FA(k.U1, i + 1, 2) - FA(k.U1, i, 2);
+      FA(k.W1, i, 1) = FA(k.W1, i, 3) + aux.x * du;
+      FA(k.QZ1, i, 1) = FA(k.QZ1, i, 3) + aux.y * du + (q1 - q0 + FA(k.QX1, i + 1, 2) - FA(k.QX1, i, 2));
     }
     // bottom row
     const int lst = k.PMLZH;
     if (i >= 1 - T && i <= MX + T) {
-      float *F[4] = {k.U, k.W, k.QX, k.QZ};
+      float *F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const float *c = (f == 1 || f == 3) ? k.abw : k.abh;
@@ -633,7 +644,7 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
     } else {
       // columns outside the bottom update keep their values; the history still
       // records them (time_loop.F90:140-153 copies whole rows)
-      float *F[4] = {k.U, k.W, k.QX, k.QZ};
+      float *F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
       for (int f = 0; f < 4; ++f) {
         float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
         float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
@@ -654,12 +665,12 @@ __global__ void k_sample(KParams k, int itile) {
   const bool okz = r.z >= lo && r.z <= hi && r.w >= 0 && r.w <= nzl;
   const bool okp = r.z >= lo && r.z <= hi && r.y >= 0 && r.y <= nzl;
   float *o = k.traces + ((size_t)(itile - 1) * k.MR + j) * 6;
-  o[0] = okx ? FA(k.U, r.x, r.y) : 0.0f;
-  o[1] = okz ? -FA(k.W, r.z, r.w) : 0.0f;
-  o[2] = okx ? FA(k.QX, r.x, r.y) : 0.0f;
-  o[3] = okz ? -FA(k.QZ, r.z, r.w) : 0.0f;
-  o[4] = okp ? FA(k.PR, r.z, r.y) : 0.0f;
-  o[5] = okp ? -FA(k.TXX, r.z, r.y) : 0.0f;
+  o[0] = okx ? FA(k.U1, r.x, r.y) : 0.0f;
+  o[1] = okz ? -FA(k.W1, r.z, r.w) : 0.0f;
+  o[2] = okx ? FA(k.QX1, r.x, r.y) : 0.0f;
+  o[3] = okz ? -FA(k.QZ1, r.z, r.w) : 0.0f;
+  o[4] = okp ? FA(k.PR1, r.z, r.y) : 0.0f;
+  o[5] = okp ? -FA(k.TXX1, r.z, r.y) : 0.0f;
 }
 
 #undef FA
@@ -694,7 +705,8 @@ struct bq_handle {
   hipStream_t stream = nullptr;
   KParams kp{};
   std::vector<void *> allocs;
-  float *fields = nullptr;
+  float *fields[2] = {nullptr, nullptr};  // two full field sets, current and next
+  int parity = 0;                         // steps done, mod 2
   std::vector<int32_t> jm_host;
   std::vector<float> tab_host;
   bool model_ok = false, pml_ok = false, src_ok = false;
@@ -771,6 +783,22 @@ int dalloc(bq_handle *h, Tp **p, size_t n) {
 
 }  // namespace
 
+// Kernel parameters of the next step: after an odd number of steps the
+// current and next field sets trade places.
+static KParams step_params(const bq_handle *h) {
+  KParams k = h->kp;
+  if (h->parity) {
+    const size_t fsz = (size_t)h->P * h->g.NZ;
+    const float **f0[8] = {&k.U0, &k.W0, &k.QX0, &k.QZ0, &k.TXX0, &k.TZZ0, &k.TXZ0, &k.PR0};
+    float **f1[8] = {&k.U1, &k.W1, &k.QX1, &k.QZ1, &k.TXX1, &k.TZZ1, &k.TXZ1, &k.PR1};
+    for (int f = 0; f < 8; ++f) {
+      *f0[f] = h->fields[1] + f * fsz;
+      *f1[f] = h->fields[0] + f * fsz;
+    }
+  }
+  return k;
+}
+
 extern "C" {
 
 int bq_create(const bq_desc *d, bq_handle **out) {
@@ -811,10 +839,15 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   const int T = g.T, NX = g.NX, NZ = g.NZ;
   int rc = 0;
   const size_t fsz = (size_t)h->P * NZ;
-  rc |= dalloc(h, &h->fields, 8 * fsz);
+  rc |= dalloc(h, &h->fields[0], 8 * fsz);
+  rc |= dalloc(h, &h->fields[1], 8 * fsz);
   if (rc) { bq_destroy(h); return rc; }
-  float **fp[8] = {&k.U, &k.W, &k.QX, &k.QZ, &k.TXX, &k.TZZ, &k.TXZ, &k.PR};
-  for (int f = 0; f < 8; ++f) *fp[f] = h->fields + f * fsz;
+  const float **f0[8] = {&k.U0, &k.W0, &k.QX0, &k.QZ0, &k.TXX0, &k.TZZ0, &k.TXZ0, &k.PR0};
+  float **f1[8] = {&k.U1, &k.W1, &k.QX1, &k.QZ1, &k.TXX1, &k.TZZ1, &k.TXZ1, &k.PR1};
+  for (int f = 0; f < 8; ++f) {
+    *f0[f] = h->fields[0] + f * fsz;
+    *f1[f] = h->fields[1] + f * fsz;
+  }
   int *jmd = nullptr;
   rc |= dalloc(h, &jmd, fsz);
   k.JM = jmd;
@@ -1060,7 +1093,6 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   if (itile_begin != h->next_step) return fail(h, -6, "steps must run in order");
   if (itile_end > h->g.MT2) return fail(h, -7, "step beyond MT2");
   if (itile_end < itile_begin) return 0;
-  const KParams &k = h->kp;
   const GridDesc &g = h->g;
   const dim3 blk(kBlock);
   const int srest = (h->ss.cum[4] + kBlock - 1) / kBlock, vrest = (h->sv.cum[4] + kBlock - 1) / kBlock;
@@ -1077,6 +1109,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     }
   }
   for (int it = itile_begin; it <= itile_end; ++it) {
+    const KParams k = step_params(h);
+    h->parity ^= 1;
     hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * kEv] : nullptr;
     if (e) (void)hipEventRecord(e[0], h->stream);
     if (sgrid) hipLaunchKernelGGL(k_stress<kRun>, dim3(sgrid), blk, 0, h->stream, k, h->ss, it, srest);
@@ -1164,14 +1198,15 @@ int bq_get_traces(bq_handle *h, float *out, int first_step, int nsteps) {
   return 0;
 }
 
-static float *field_ptr(bq_handle *h, int id) {
+// The latest values of a field: the current set of the next step.
+static float *field_ptr(bq_handle *h, int id, int which) {
   if (id < 0 || id >= 8) return nullptr;
-  return h->fields + (size_t)id * h->P * h->g.NZ;
+  return h->fields[which] + (size_t)id * h->P * h->g.NZ;
 }
 
 int bq_get_field(bq_handle *h, int field_id, float *host) {
   if (!h || !host) return fail(h, -1, "null argument");
-  float *d = field_ptr(h, field_id);
+  float *d = field_ptr(h, field_id, h->parity);
   if (!d) return fail(h, -8, "bad field id");
   HIPCHK(h, hipMemcpy2DAsync(host, (size_t)h->g.NX * 4, d, (size_t)h->P * 4, (size_t)h->g.NX * 4,
                              h->g.NZ, hipMemcpyDeviceToHost, h->stream));
@@ -1181,10 +1216,13 @@ int bq_get_field(bq_handle *h, int field_id, float *host) {
 
 int bq_set_field(bq_handle *h, int field_id, const float *host) {
   if (!h || !host) return fail(h, -1, "null argument");
-  float *d = field_ptr(h, field_id);
-  if (!d) return fail(h, -8, "bad field id");
-  HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->P * 4, host, (size_t)h->g.NX * 4, (size_t)h->g.NX * 4,
-                             h->g.NZ, hipMemcpyHostToDevice, h->stream));
+  // both sets: cells no step writes must hold the same value in each
+  for (int which = 0; which < 2; ++which) {
+    float *d = field_ptr(h, field_id, which);
+    if (!d) return fail(h, -8, "bad field id");
+    HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->P * 4, host, (size_t)h->g.NX * 4, (size_t)h->g.NX * 4,
+                               h->g.NZ, hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
