@@ -164,17 +164,17 @@ class Simulation:
         self._chk(self.lib.bq_enable_timing(self.h, int(on)))
 
     def kernel_times(self) -> dict:
-        """ms per step of each kernel: k_stress, k_velocity, k_edges, k_sample."""
-        out = np.zeros(4, dtype=np.float64)
+        """ms per step of each launch: k_step (fused interior + boundary stress +
+        previous step's samples), k_velocity (boundary velocity), k_edges."""
+        out = np.zeros(3, dtype=np.float64)
         self._chk(self.lib.bq_kernel_times(self.h, _ptr(out)))
-        return dict(zip(("k_stress", "k_velocity", "k_edges", "k_sample"), map(float, out)))
+        return dict(zip(("k_step", "k_velocity", "k_edges"), map(float, out)))
 
     def kernel_cells(self) -> dict:
-        """Cells per launch of k_stress and k_velocity, and of their interior runs."""
-        out = np.zeros(4, dtype=np.int64)
+        """Cells updated per launch of k_fused, k_stress and k_velocity."""
+        out = np.zeros(3, dtype=np.int64)
         self._chk(self.lib.bq_kernel_cells(self.h, _ptr(out)))
-        return dict(zip(("k_stress", "k_velocity", "k_stress_interior", "k_velocity_interior"),
-                        map(int, out)))
+        return dict(zip(("k_fused", "k_stress", "k_velocity"), map(int, out)))
 
     def field(self, name: str) -> np.ndarray:
         out = np.zeros((self.nz, self.nx), dtype=np.float32)

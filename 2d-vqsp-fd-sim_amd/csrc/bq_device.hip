@@ -374,8 +374,10 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
 // four halo lanes. The arithmetic is the same expression as the per-cell path.
 
 struct Split {
-  int fx0, ntx, fz0, fz1, ntz, nr;  // interior: ntx 64-column strips from fx0, rows [fz0, fz1),
-                                    // in ntz blocks of 4 runs of nr rows
+  // interior runs: up to 3 rectangles of 64-column strips, [rx0 + 64*ntx) x [rz0, rz1),
+  // each in blocks of 64 columns x 4 runs of nr rows (rblk blocks, rcum cumulative)
+  int nr, nrect;
+  int qx0[3], qntx[3], qz0[3], qz1[3], qblk[3], qcum[4];
   int rx0[4], rx1[4], rz0[4], rz1[4];      // remaining cells: four rectangles [x0,x1) x [z0,z1)
   int cum[5];                              // cumulative cell counts of the rectangles
 };
@@ -411,13 +413,13 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 }
 
 template <int NR>
-__device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, int tile, int wv,
-                                           int lane, int itile) {
-  const int tx = tile % sp.ntx, tz = tile / sp.ntx;
-  const int c = sp.fx0 + tx * 64;
+__device__ __forceinline__ void stress_run(const KParams &k, int x0, int ntx, int z0, int z1,
+                                           int tile, int wv, int lane, int itile) {
+  const int tx = tile % ntx, tz = tile / ntx;
+  const int c = x0 + tx * 64;
   const int i = c + lane;
-  const int l0 = sp.fz0 + (tz * 4 + wv) * NR;
-  if (l0 >= sp.fz1) return;
+  const int l0 = z0 + (tz * 4 + wv) * NR;
+  if (l0 >= z1) return;
   const int P = k.P;
   const int ix = i + k.OFF;
   const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
@@ -449,7 +451,7 @@ __device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, in
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
     const int l = l0 + j;
-    if (l >= sp.fz1) break;
+    if (l >= z1) break;
     const int r = l * P;
     const float qx0 = QX[r + ix];
     const XNb un = xneighbours(u[j + 2], U[r + hx], lane);
@@ -485,13 +487,13 @@ __device__ __forceinline__ void stress_run(const KParams &k, const Split &sp, in
 }
 
 template <int NR>
-__device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, int tile, int wv,
-                                             int lane) {
-  const int tx = tile % sp.ntx, tz = tile / sp.ntx;
-  const int c = sp.fx0 + tx * 64;
+__device__ __forceinline__ void velocity_run(const KParams &k, int x0, int ntx, int z0, int z1,
+                                             int tile, int wv, int lane) {
+  const int tx = tile % ntx, tz = tile / ntx;
+  const int c = x0 + tx * 64;
   const int i = c + lane;
-  const int l0 = sp.fz0 + (tz * 4 + wv) * NR;
-  if (l0 >= sp.fz1) return;
+  const int l0 = z0 + (tz * 4 + wv) * NR;
+  if (l0 >= z1) return;
   const int P = k.P;
   const int ix = i + k.OFF;
   const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
@@ -524,7 +526,7 @@ __device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, 
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
     const int l = l0 + j;
-    if (l >= sp.fz1) break;
+    if (l >= z1) break;
     const int r = l * P;
     const float txx0 = TXX[r + ix];
     const float xz0 = xz[j + 2], p0 = pp[j + 2];
@@ -547,6 +549,7 @@ __device__ __forceinline__ void velocity_run(const KParams &k, const Split &sp, 
 
 constexpr int kBlock = 256;  // 4 wavefronts
 constexpr int kRun = 4;      // rows per wavefront run
+constexpr int kFR = 128;     // rows per fused block
 
 __device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l) {
   if (g >= sp.cum[4]) return false;
@@ -559,20 +562,16 @@ __device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l
   return true;
 }
 
-// One launch per half-step: the first nrest blocks take the remaining cells
-// one per thread (per-cell path), the others take the interior runs, so the
-// slow boundary cells overlap with the interior instead of trailing it.
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_stress(KParams k, Split sp, int itile, int nrest) {
-  if ((int)blockIdx.x < nrest) {
-    int i, l;
-    if (rest_cell(sp, blockIdx.x * kBlock + threadIdx.x, &i, &l)) stress_cell(k, i, l, itile);
-    return;
-  }
-  const int b = xcd_tile(blockIdx.x - nrest, gridDim.x - nrest);
-  stress_run<NR>(k, sp, b, threadIdx.x >> 6, threadIdx.x & 63, itile);
+// Block b of the interior runs -> rectangle and tile of that rectangle.
+__device__ __forceinline__ int run_rect(const Split &sp, int b, int *tile) {
+  int r = 0;
+  while (r + 1 < sp.nrect && b >= sp.qcum[r + 1]) ++r;
+  *tile = b - sp.qcum[r];
+  return r;
 }
 
+// The velocity half-step of the cells outside the fused blocks: the first
+// nrest blocks take cells one per thread (per-cell path), the others the runs.
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nrest) {
   if ((int)blockIdx.x < nrest) {
@@ -580,8 +579,267 @@ __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nr
     if (rest_cell(sp, blockIdx.x * kBlock + threadIdx.x, &i, &l)) velocity_cell(k, i, l);
     return;
   }
-  const int b = xcd_tile(blockIdx.x - nrest, gridDim.x - nrest);
-  velocity_run<NR>(k, sp, b, threadIdx.x >> 6, threadIdx.x & 63);
+  int tile;
+  const int r = run_rect(sp, xcd_tile(blockIdx.x - nrest, gridDim.x - nrest), &tile);
+  velocity_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+                   threadIdx.x & 63);
+}
+
+// -------------------------------------------------------------- fused step
+// Interior cells far enough from every boundary that both half-steps use only
+// their 4th-order formulas are advanced by one kernel that does the stress
+// and the velocity half-step together. A block owns kFW columns x NR rows and
+// marches down them: at row s it forms the new stresses of row s (its own
+// columns and two columns on either side), then the new velocities of row
+// s-2, whose stencils reach new stresses two rows and two columns away. The
+// stresses of the two rows above and below the block and of the side columns
+// are recomputed from the current fields, not stored; the rows a block owns
+// are written once. Old velocity rows and new stress rows pass through LDS
+// rings; the block's own column keeps its z-windows in registers. Fields are
+// read from the current set and written to the next, so one step moves
+// 8 fields in, 8 fields out and one material index per cell.
+constexpr int kFW = 256;       // columns per block, one thread each
+constexpr int kLW = kFW + 8;   // old-velocity LDS row: columns c-4 .. c+kFW+3
+constexpr int kSW = kFW + 4;   // new-stress LDS row: columns c-2 .. c+kFW+1
+
+struct Fused {
+  int vx0, nfx, vz0, vz1, nfz;  // blocks: columns vx0 + kFW*bx, rows vz0 + NR*bz (clipped to vz1)
+};
+
+// New TXX, TZZ, TXZ, PRES at LDS column q (global column i) of row s, from the
+// old-velocity rings (same expressions as stress_run).
+__device__ __forceinline__ void fused_stress(const KParams &k, const float (*sU)[kLW],
+                                             const float (*sW)[kLW], const float (*sZ)[kLW],
+                                             const float (*sQ)[kLW], int s, int q, int i,
+                                             float txx, float tzz, float txz, float pr, int jm,
+                                             int itile, float *o) {
+  const int u0s = s % 5, w0s = s % 6, qs = s & 3;
+  const float um2 = sU[(s - 2) % 5][q], um1 = sU[(s - 1) % 5][q], u0 = sU[u0s][q];
+  const float up1 = sU[(s + 1) % 5][q];
+  const float uxm1 = sU[u0s][q - 1], uxp1 = sU[u0s][q + 1], uxp2 = sU[u0s][q + 2];
+  const float wm1 = sW[(s - 1) % 6][q], w0 = sW[w0s][q], wp1 = sW[(s + 1) % 6][q];
+  const float wp2 = sW[(s + 2) % 6][q];
+  const float wxm2 = sW[w0s][q - 2], wxm1 = sW[w0s][q - 1], wxp1 = sW[w0s][q + 1];
+  const float zm1 = sZ[(s - 1) % 6][q], z0 = sZ[w0s][q], zp1 = sZ[(s + 1) % 6][q];
+  const float zp2 = sZ[(s + 2) % 6][q];
+  const float qxm1 = sQ[qs][q - 1], qx0 = sQ[qs][q], qxp1 = sQ[qs][q + 1], qxp2 = sQ[qs][q + 2];
+  const float AH = k.AH, BH = k.BH, BC = k.BC, C = k.CH, DT = k.DT;
+  const float4 s0 = k.stab[3 * jm], s1 = k.stab[3 * jm + 1], s2 = k.stab[3 * jm + 2];
+  const float exz = AH * (up1 - um2 + wxp1 - wxm2) + BH * (u0 - um1 + w0 - wxm1);
+  const float exx = AH * (uxp2 - uxm1) + BH * (uxp1 - u0);
+  const float qxx = AH * (qxp2 - qxm1) + BH * (qxp1 - qx0);
+  float ezz = (C * (wp1 - w0));
+  float qzz = (C * (zp1 - z0));
+  ezz = AH * (wp2 - wm1) + BC * ezz;
+  qzz = AH * (zp2 - zm1) + BC * qzz;
+  const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
+  const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
+  const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
+  o[2] = txz + DT * (s0.x * exz);
+  if (i == k.LIN_X && s == k.LIN_Z) {
+    const float2 ds = k.dsrc[itile - 1];
+    o[0] = txx + DT * (rxx - ds.x);
+    o[1] = tzz + DT * (rzz - ds.x);
+    o[3] = pr + DT * (rpp + ds.y);
+  } else {
+    o[0] = txx + DT * rxx;
+    o[1] = tzz + DT * rzz;
+    o[3] = pr + DT * rpp;
+  }
+}
+
+struct FusedLds {
+  float sU[5][kLW], sW[6][kLW], sZ[6][kLW], sQ[4][kLW];
+  float nX[3][kSW], nXZ[3][kSW], nP[3][kSW];
+};
+
+template <int NR>
+__device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, int b, int itile,
+                                            FusedLds &L) {
+  auto &sU = L.sU;
+  auto &sW = L.sW;
+  auto &sZ = L.sZ;
+  auto &sQ = L.sQ;
+  auto &nX = L.nX;
+  auto &nXZ = L.nXZ;
+  auto &nP = L.nP;
+  const int t = threadIdx.x;
+  const int bx = b % fu.nfx, bz = b / fu.nfx;
+  const int c = fu.vx0 + bx * kFW;
+  const int a = fu.vz0 + bz * NR;
+  const int e = min(a + NR, fu.vz1);
+  const int P = k.P, OFF = k.OFF;
+  const int i = c + t;
+  const int gx = i + OFF;
+  const bool hl = t < 8;                         // loads one extra old-velocity column
+  const int hq = t < 4 ? t : kFW + t;            // its LDS column (column c-4+hq)
+  const int hgx = c - 4 + hq + OFF;
+  const bool hs = t < 4;                         // forms one extra column of new stresses
+  const int hcol = t < 2 ? c - 2 + t : c + kFW + (t - 2);
+  const int hgs = hcol + OFF;
+  const float *__restrict__ U = k.U0;
+  const float *__restrict__ W = k.W0;
+  const float *__restrict__ QX = k.QX0;
+  const float *__restrict__ QZ = k.QZ0;
+  const int *__restrict__ JM = k.JM;
+  // prefill the rings: U rows a-4..a-2, W and QZ rows a-3..a-1
+  for (int r = a - 4; r <= a - 2; ++r) {
+    sU[r % 5][t + 4] = U[r * P + gx];
+    sW[(r + 1) % 6][t + 4] = W[(r + 1) * P + gx];
+    sZ[(r + 1) % 6][t + 4] = QZ[(r + 1) * P + gx];
+    if (hl) {
+      sU[r % 5][hq] = U[r * P + hgx];
+      sW[(r + 1) % 6][hq] = W[(r + 1) * P + hgx];
+      sZ[(r + 1) % 6][hq] = QZ[(r + 1) * P + hgx];
+    }
+  }
+  float wxz0 = 0, wxz1 = 0, wxz2 = 0, wxz3 = 0;             // new TXZ rows s-3..s
+  float wzz0 = 0, wzz1 = 0, wzz2 = 0, wzz3 = 0, wzz4 = 0;   // new TZZ rows s-4..s
+  float wp0 = 0, wp1 = 0, wp2 = 0, wp3 = 0, wp4 = 0;        // new PRES rows s-4..s
+  int jm1 = 0, jm2 = 0, jm0 = 0;                            // JM rows s-2, s-1, s
+  // values of row s, loaded one iteration ahead so that the loads of the next
+  // row are in flight while this row is computed
+  float pu, pw, pz, pq, fhu = 0, fhw = 0, fhz = 0, fhq = 0;
+  float ftxx, ftzz, ftxz, fpr, fhtxx = 0, fhtzz = 0, fhtxz = 0, fhpr = 0;
+  int fjm, fhjm = 0;
+  auto fetch = [&](int s) {
+    const int r = s * P;
+    pu = U[r + P + gx]; pw = W[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
+    if (hl) { fhu = U[r + P + hgx]; fhw = W[r + 2 * P + hgx]; fhz = QZ[r + 2 * P + hgx]; fhq = QX[r + hgx]; }
+    ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
+    fjm = JM[r + gx] - 1;
+    if (hs) {
+      fhtxx = k.TXX0[r + hgs]; fhtzz = k.TZZ0[r + hgs]; fhtxz = k.TXZ0[r + hgs];
+      fhpr = k.PR0[r + hgs]; fhjm = JM[r + hgs] - 1;
+    }
+  };
+  fetch(a - 2);
+  for (int s = a - 2; s <= e + 1; ++s) {
+    const int r = s * P;
+    // ---- old velocities entering the rings, old stresses and index of row s
+    sU[(s + 1) % 5][t + 4] = pu;
+    sW[(s + 2) % 6][t + 4] = pw;
+    sZ[(s + 2) % 6][t + 4] = pz;
+    sQ[s & 3][t + 4] = pq;
+    if (hl) {
+      sU[(s + 1) % 5][hq] = fhu;
+      sW[(s + 2) % 6][hq] = fhw;
+      sZ[(s + 2) % 6][hq] = fhz;
+      sQ[s & 3][hq] = fhq;
+    }
+    const float otxx = ftxx, otzz = ftzz, otxz = ftxz, opr = fpr;
+    jm2 = jm1;
+    jm1 = jm0;
+    jm0 = fjm;
+    const float htxx = fhtxx, htzz = fhtzz, htxz = fhtxz, hpr = fhpr;
+    const int hjm = fhjm;
+    if (s <= e) fetch(s + 1);
+    __syncthreads();
+    // ---- new stresses of row s
+    float o[4];
+    fused_stress(k, sU, sW, sZ, sQ, s, t + 4, i, otxx, otzz, otxz, opr, jm0, itile, o);
+    const int ns = s % 3;
+    nX[ns][t + 2] = o[0];
+    nXZ[ns][t + 2] = o[2];
+    nP[ns][t + 2] = o[3];
+    if (s >= a && s < e) {
+      k.TXX1[r + gx] = o[0];
+      k.TZZ1[r + gx] = o[1];
+      k.TXZ1[r + gx] = o[2];
+      k.PR1[r + gx] = o[3];
+    }
+    wxz0 = wxz1; wxz1 = wxz2; wxz2 = wxz3; wxz3 = o[2];
+    wzz0 = wzz1; wzz1 = wzz2; wzz2 = wzz3; wzz3 = wzz4; wzz4 = o[1];
+    wp0 = wp1; wp1 = wp2; wp2 = wp3; wp3 = wp4; wp4 = o[3];
+    if (hs) {
+      float ho[4];
+      fused_stress(k, sU, sW, sZ, sQ, s, hcol - (c - 4), hcol, htxx, htzz, htxz, hpr, hjm, itile, ho);
+      const int hn = hcol - (c - 2);
+      nX[ns][hn] = ho[0];
+      nXZ[ns][hn] = ho[2];
+      nP[ns][hn] = ho[3];
+    }
+    __syncthreads();
+    // ---- new velocities of row l = s-2 (INH_QX_INT, INH_V_UV_INT, INH_QZ_INT,
+    //      INH_V_W_INT: the expressions of velocity_run)
+    const int l = s - 2;
+    if (l >= a && l < e) {
+      const int sl = l % 3, rl = l * P;
+      const float A = k.A, B = k.B;
+      const float xm2 = nX[sl][t], xm1 = nX[sl][t + 1], x0 = nX[sl][t + 2], xp1 = nX[sl][t + 3];
+      const float xzm1 = nXZ[sl][t + 1], xz0 = nXZ[sl][t + 2], xzp1 = nXZ[sl][t + 3];
+      const float xzp2 = nXZ[sl][t + 4];
+      const float pm2 = nP[sl][t], pm1 = nP[sl][t + 1], p0 = nP[sl][t + 2], pp1 = nP[sl][t + 3];
+      const float q = sQ[l & 3][t + 4], u = sU[l % 5][t + 4];
+      const float qz = sZ[l % 6][t + 4], w = sW[l % 6][t + 4];
+      const float4 v0 = k.vtab[3 * jm2], v1 = k.vtab[3 * jm2 + 1], v2 = k.vtab[3 * jm2 + 2];
+      // TXZ(l-1..l+2) = wxz0..wxz3, TZZ and PRES (l-2..l+1) = w*0..w*3 at column i
+      const float sx = A * (xp1 - xm2 + wxz3 - wxz0) + B * (x0 - xm1 + wxz2 - wxz1);
+      const float px = A * (pp1 - pm2) + B * (p0 - pm1);
+      k.QX1[rl + gx] = v0.x * q - v0.y * sx - v0.z * px;
+      k.U1[rl + gx] = u + v0.w * q + v1.x * sx + v1.y * px;
+      const float sz = A * (wzz3 - wzz0 + xzp2 - xzm1) + B * (wzz2 - wzz1 + xzp1 - xz0);
+      const float pz = A * (wp3 - wp0) + B * (wp2 - wp1);
+      k.QZ1[rl + gx] = v1.z * qz - v1.w * sz - v2.x * pz;
+      k.W1[rl + gx] = w + v2.y * qz + v2.z * sz + v2.w * pz;
+    }
+  }
+}
+
+// Receiver samples of the fields a step left behind (store_seismograms.F90:58-83,
+// store_seismograms_pressure.F90:50-56); f points at the set holding them.
+__device__ __forceinline__ void sample_receivers(const KParams &k, int j, int step, bool cur) {
+  if (j >= k.MR) return;
+  const int4 r = k.rec[j];
+  const int nzl = k.PMLZH, lo = k.PMLL, hi = k.PMLL + k.NX - 1;
+  const bool okx = r.x >= lo && r.x <= hi && r.y >= 0 && r.y <= nzl;
+  const bool okz = r.z >= lo && r.z <= hi && r.w >= 0 && r.w <= nzl;
+  const bool okp = r.z >= lo && r.z <= hi && r.y >= 0 && r.y <= nzl;
+  const float *U = cur ? k.U0 : k.U1, *W = cur ? k.W0 : k.W1;
+  const float *QX = cur ? k.QX0 : k.QX1, *QZ = cur ? k.QZ0 : k.QZ1;
+  const float *PR = cur ? k.PR0 : k.PR1, *TXX = cur ? k.TXX0 : k.TXX1;
+  float *o = k.traces + ((size_t)(step - 1) * k.MR + j) * 6;
+  o[0] = okx ? FA(U, r.x, r.y) : 0.0f;
+  o[1] = okz ? -FA(W, r.z, r.w) : 0.0f;
+  o[2] = okx ? FA(QX, r.x, r.y) : 0.0f;
+  o[3] = okz ? -FA(QZ, r.z, r.w) : 0.0f;
+  o[4] = okp ? FA(PR, r.z, r.y) : 0.0f;
+  o[5] = okp ? -FA(TXX, r.z, r.y) : 0.0f;
+}
+
+// First launch of a step: fused interior blocks (long-running, first), then
+// the stress half-step of every other cell (per-cell blocks and runs), then
+// the receiver samples of the previous step, taken from the current set,
+// which this launch only reads. All parts are independent of each other.
+struct Launch1 {
+  int nfused, nrest, nruns, nsample, sample_step;
+};
+
+template <int NR, int FR>
+__global__ __launch_bounds__(kBlock) void k_step(KParams k, Fused fu, Split sp, Launch1 ln,
+                                                 int itile) {
+  __shared__ FusedLds L;
+  int b = blockIdx.x;
+  if (b < ln.nfused) {
+    fused_block<FR>(k, fu, xcd_tile(b, ln.nfused), itile, L);
+    return;
+  }
+  b -= ln.nfused;
+  if (b < ln.nrest) {
+    int i, l;
+    if (rest_cell(sp, b * kBlock + threadIdx.x, &i, &l)) stress_cell(k, i, l, itile);
+    return;
+  }
+  b -= ln.nrest;
+  if (b < ln.nruns) {
+    int tile;
+    const int r = run_rect(sp, xcd_tile(b, ln.nruns), &tile);
+    stress_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+                   threadIdx.x & 63, itile);
+    return;
+  }
+  b -= ln.nruns;
+  sample_receivers(k, b * kBlock + threadIdx.x, ln.sample_step, true);
 }
 
 // ------------------------------------------------------------------- edges
@@ -655,22 +913,9 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
   }
 }
 
-// Receiver samples (store_seismograms.F90:58-83, store_seismograms_pressure.F90:50-56)
+// Receiver samples of the last step of a run, from the set it wrote.
 __global__ void k_sample(KParams k, int itile) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k.MR) return;
-  const int4 r = k.rec[j];
-  const int nzl = k.PMLZH, lo = k.PMLL, hi = k.PMLL + k.NX - 1;
-  const bool okx = r.x >= lo && r.x <= hi && r.y >= 0 && r.y <= nzl;
-  const bool okz = r.z >= lo && r.z <= hi && r.w >= 0 && r.w <= nzl;
-  const bool okp = r.z >= lo && r.z <= hi && r.y >= 0 && r.y <= nzl;
-  float *o = k.traces + ((size_t)(itile - 1) * k.MR + j) * 6;
-  o[0] = okx ? FA(k.U1, r.x, r.y) : 0.0f;
-  o[1] = okz ? -FA(k.W1, r.z, r.w) : 0.0f;
-  o[2] = okx ? FA(k.QX1, r.x, r.y) : 0.0f;
-  o[3] = okz ? -FA(k.QZ1, r.z, r.w) : 0.0f;
-  o[4] = okp ? FA(k.PR1, r.z, r.y) : 0.0f;
-  o[5] = okp ? -FA(k.TXX1, r.z, r.y) : 0.0f;
+  sample_receivers(k, blockIdx.x * blockDim.x + threadIdx.x, itile, false);
 }
 
 #undef FA
@@ -714,39 +959,48 @@ struct bq_handle {
   int mr = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;
-  double ktime[4] = {0, 0, 0, 0};
+  double ktime[3] = {0, 0, 0};
   int timed_steps = 0;
   Split ss{}, sv{};
+  Fused fu{};
+  long long fused_cells = 0, stress_cells = 0, velocity_cells = 0;
   std::string err;
 };
 
 namespace {
 
 // Partition of the cells a half-step updates, x in [PMLL, PMLXH] and rows
-// [z_lo, z_hi], into 64-column fast tiles over [fx_lo, fx_hi] x [fz0, fz1)
-// and four rectangles of remaining cells. The fast paths read rows up to
-// fz1+2 (z-stencil plus the one-row load-ahead) and columns fx_lo-2 ..
-// fx_hi+2; with fz1 <= MZ'-1, TPML >= 1 and 4 <= fx_lo, fx_hi <= MX-2 these
-// stay inside the grid.
-Split make_split(const GridDesc &g, int z_lo, int z_hi, int fx_lo, int fx_hi, int fz0, int fz1,
-                 int nr) {
+// [z_lo, z_hi]: the interior runs over the given rectangles (64-column strips
+// from x0 to x1, rows [z0, z1)) and per-cell blocks for every cell outside the
+// base rectangle [bx0, bx1) x [bz0, bz1) that encloses the runs and the fused
+// tiles. The runs read rows up to z1+2 and columns x0-2 .. x1+1; with
+// z1 <= MZ'-1, TPML >= 1 and 4 <= x0, x1 <= MX-1 these stay inside the grid.
+struct RunRect { int x0, x1, z0, z1; };
+
+Split make_split(const GridDesc &g, int z_lo, int z_hi, int bx0, int bx1, int bz0, int bz1,
+                 const RunRect *rr, int nrr, int nr) {
   Split s{};
   s.nr = nr;
-  s.fx0 = fx_lo;
-  s.ntx = fx_hi >= fx_lo ? (fx_hi - fx_lo + 1) / 64 : 0;
-  s.fz0 = fz0;
-  s.fz1 = fz1;
-  s.ntz = (fz1 > fz0 && s.ntx > 0) ? (fz1 - fz0 + 4 * nr - 1) / (4 * nr) : 0;
-  if (s.ntz == 0) s.ntx = 0;
-  const int fxe = s.fx0 + 64 * s.ntx;  // first column after the fast tiles
-  if (s.ntx * s.ntz == 0) {  // everything on the per-cell path
+  s.nrect = 0;
+  s.qcum[0] = 0;
+  for (int q = 0; q < nrr; ++q) {
+    const int ntx = rr[q].x1 > rr[q].x0 ? (rr[q].x1 - rr[q].x0) / 64 : 0;
+    const int nz = rr[q].z1 > rr[q].z0 ? (rr[q].z1 - rr[q].z0 + 4 * nr - 1) / (4 * nr) : 0;
+    if (ntx * nz == 0) continue;
+    const int k = s.nrect++;
+    s.qx0[k] = rr[q].x0; s.qntx[k] = ntx; s.qz0[k] = rr[q].z0; s.qz1[k] = rr[q].z1;
+    s.qblk[k] = ntx * nz;
+    s.qcum[k + 1] = s.qcum[k] + s.qblk[k];
+  }
+  for (int k = s.nrect; k < 3; ++k) s.qcum[k + 1] = s.qcum[k];
+  if (bx1 <= bx0 || bz1 <= bz0) {  // everything on the per-cell path
     s.rx0[0] = g.PMLL; s.rx1[0] = g.PMLXH + 1; s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
     for (int r = 1; r < 4; ++r) { s.rx0[r] = s.rx1[r] = 0; s.rz0[r] = s.rz1[r] = 0; }
   } else {
-    s.rx0[0] = g.PMLL; s.rx1[0] = s.fx0;       s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
-    s.rx0[1] = fxe;    s.rx1[1] = g.PMLXH + 1; s.rz0[1] = z_lo; s.rz1[1] = z_hi + 1;
-    s.rx0[2] = s.fx0;  s.rx1[2] = fxe;         s.rz0[2] = z_lo; s.rz1[2] = fz0;
-    s.rx0[3] = s.fx0;  s.rx1[3] = fxe;         s.rz0[3] = fz1;  s.rz1[3] = z_hi + 1;
+    s.rx0[0] = g.PMLL; s.rx1[0] = bx0;         s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
+    s.rx0[1] = bx1;    s.rx1[1] = g.PMLXH + 1; s.rz0[1] = z_lo; s.rz1[1] = z_hi + 1;
+    s.rx0[2] = bx0;    s.rx1[2] = bx1;         s.rz0[2] = z_lo; s.rz1[2] = bz0;
+    s.rx0[3] = bx0;    s.rx1[3] = bx1;         s.rz0[3] = bz1;  s.rz1[3] = z_hi + 1;
   }
   s.cum[0] = 0;
   for (int r = 0; r < 4; ++r) {
@@ -894,14 +1148,45 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   k.dsrc = ds;
   if (rc) { int r = rc; bq_destroy(h); return r < 0 ? r : -101; }
   if (hipStreamSynchronize(h->stream) != hipSuccess) { bq_destroy(h); return -100; }
-  // interior tiles start at the first column >= 4 whose address is 128-byte
-  // aligned, so a wavefront's 64 values of a row are exactly two cache lines
-  int fx = 4;
+  // Interior tiles start at the first column >= 5 whose address is 128-byte
+  // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
+  int fx = 5;
   while ((fx + k.OFF) % 32 != 0) ++fx;
-  // stress: rows 2..PMLZH; interior rows 5..MZ'-2 (rows 2-4 write the image rows)
-  h->ss = make_split(g, 2, g.PMLZH, fx, g.MX - 2, 5, g.MZ - 1, kRun);
-  // velocity: rows 2..PMLZH-1; interior rows 2..MZ'-2
-  h->sv = make_split(g, 2, g.PMLZH - 1, fx, g.MX - 2, 2, g.MZ - 1, kRun);
+  const int ntx = g.MX - 2 >= fx ? (g.MX - 2 - fx + 1) / 64 : 0;
+  const int fxe = fx + 64 * ntx;  // runs cover columns [fx, fxe), inside [5, MX-2]
+  // Fused tiles: columns [fx, vx1), vx1 <= MX-3, rows [5, MZ'-3): their new
+  // stresses reach two rows and columns further, still inside [3, MX-2] x
+  // [3, MZ'-2], where both half-steps use the interior formulas.
+  Fused &fu = h->fu;
+  fu.vx0 = fx;
+  fu.nfx = g.MX - 3 > fx ? (g.MX - 3 - fx) / kFW : 0;
+  fu.vz0 = 5;
+  fu.vz1 = g.MZ - 3;
+  fu.nfz = (fu.nfx > 0 && fu.vz1 > fu.vz0) ? (fu.vz1 - fu.vz0 + kFR - 1) / kFR : 0;
+  if (fu.nfz == 0) fu.nfx = 0;
+  const int vx1 = fu.nfx ? fx + kFW * fu.nfx : fx;
+  h->fused_cells = (long long)kFW * fu.nfx * (fu.vz1 - fu.vz0) * (fu.nfz ? 1 : 0);
+  // stress half-step outside the fused tiles: runs on [vx1, fxe) x [5, MZ'-1)
+  // and [fx, vx1) x [MZ'-3, MZ'-1); rows 2-4 (image rows), the sides and the
+  // bottom rows on the per-cell path
+  if (fu.nfx) {
+    const RunRect rs[2] = {{vx1, fxe, 5, g.MZ - 1}, {fx, vx1, fu.vz1, g.MZ - 1}};
+    h->ss = make_split(g, 2, g.PMLZH, fx, fxe, 5, g.MZ - 1, rs, 2, kRun);
+    const RunRect rv[3] = {{vx1, fxe, 2, g.MZ - 1}, {fx, vx1, 2, 5}, {fx, vx1, fu.vz1, g.MZ - 1}};
+    h->sv = make_split(g, 2, g.PMLZH - 1, fx, fxe, 2, g.MZ - 1, rv, 3, kRun);
+  } else {
+    const RunRect rs[1] = {{fx, fxe, 5, g.MZ - 1}};
+    h->ss = make_split(g, 2, g.PMLZH, fx, fxe, 5, g.MZ - 1, rs, 1, kRun);
+    const RunRect rv[1] = {{fx, fxe, 2, g.MZ - 1}};
+    h->sv = make_split(g, 2, g.PMLZH - 1, fx, fxe, 2, g.MZ - 1, rv, 1, kRun);
+  }
+  auto runs_cells = [](const Split &sp) {
+    long long n = sp.cum[4];
+    for (int q = 0; q < sp.nrect; ++q) n += 64LL * sp.qntx[q] * (sp.qz1[q] - sp.qz0[q]);
+    return n;
+  };
+  h->stress_cells = runs_cells(h->ss);
+  h->velocity_cells = runs_cells(h->sv);
   *out = h;
   return 0;
 }
@@ -1095,11 +1380,16 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   if (itile_end < itile_begin) return 0;
   const GridDesc &g = h->g;
   const dim3 blk(kBlock);
-  const int srest = (h->ss.cum[4] + kBlock - 1) / kBlock, vrest = (h->sv.cum[4] + kBlock - 1) / kBlock;
-  const int sgrid = srest + h->ss.ntx * h->ss.ntz, vgrid = vrest + h->sv.ntx * h->sv.ntz;
+  Launch1 ln{};
+  ln.nfused = h->fu.nfx * h->fu.nfz;
+  ln.nrest = (h->ss.cum[4] + kBlock - 1) / kBlock;
+  ln.nruns = h->ss.qcum[3];
+  const int vrest = (h->sv.cum[4] + kBlock - 1) / kBlock;
+  const int vgrid = vrest + h->sv.qcum[3];
+  const int nsb = (h->mr + kBlock - 1) / kBlock;
   const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
   const int nsteps = itile_end - itile_begin + 1;
-  constexpr int kEv = 5;
+  constexpr int kEv = 4;
   if (h->timing) {
     const size_t need = (size_t)nsteps * kEv;
     while (h->ev.size() < need) {
@@ -1111,22 +1401,26 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   for (int it = itile_begin; it <= itile_end; ++it) {
     const KParams k = step_params(h);
     h->parity ^= 1;
+    // the samples of step it-1 ride along with the first launch of step it
+    ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
+    ln.sample_step = it - 1;
+    const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
     hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * kEv] : nullptr;
     if (e) (void)hipEventRecord(e[0], h->stream);
-    if (sgrid) hipLaunchKernelGGL(k_stress<kRun>, dim3(sgrid), blk, 0, h->stream, k, h->ss, it, srest);
+    if (g1) hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), blk, 0, h->stream, k, h->fu, h->ss, ln, it);
     if (e) (void)hipEventRecord(e[1], h->stream);
     if (vgrid) hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
     if (e) (void)hipEventRecord(e[2], h->stream);
     hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
     if (e) (void)hipEventRecord(e[3], h->stream);
-    if (k.MR > 0) hipLaunchKernelGGL(k_sample, dim3((k.MR + 255) / 256), dim3(256), 0, h->stream, k, it);
-    if (e) (void)hipEventRecord(e[4], h->stream);
+    if (it == itile_end && k.MR > 0)
+      hipLaunchKernelGGL(k_sample, dim3(nsb), dim3(kBlock), 0, h->stream, k, it);
   }
   HIPCHK(h, hipGetLastError());
   h->next_step = itile_end + 1;
   if (h->timing) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    double acc[kEv - 1] = {0, 0, 0, 0};
+    double acc[kEv - 1] = {0, 0, 0};
     for (int s = 0; s < nsteps; ++s) {
       for (int q = 0; q < kEv - 1; ++q) {
         float ms = 0;
@@ -1140,19 +1434,18 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   return 0;
 }
 
-int bq_kernel_cells(bq_handle *h, long long out[4]) {
+int bq_kernel_cells(bq_handle *h, long long out[3]) {
   if (!h || !out) return -1;
-  out[0] = 64LL * h->ss.ntx * (h->ss.fz1 - h->ss.fz0) + h->ss.cum[4];
-  out[1] = 64LL * h->sv.ntx * (h->sv.fz1 - h->sv.fz0) + h->sv.cum[4];
-  out[2] = 64LL * h->ss.ntx * (h->ss.fz1 - h->ss.fz0);
-  out[3] = 64LL * h->sv.ntx * (h->sv.fz1 - h->sv.fz0);
+  out[0] = h->fused_cells;
+  out[1] = h->stress_cells;
+  out[2] = h->velocity_cells;
   return 0;
 }
 
-int bq_kernel_times(bq_handle *h, double out[4]) {
+int bq_kernel_times(bq_handle *h, double out[3]) {
   if (!h || !out) return -1;
   if (!h->timed_steps) return fail(h, -6, "no timed run");
-  for (int q = 0; q < 4; ++q) out[q] = h->ktime[q];
+  for (int q = 0; q < 3; ++q) out[q] = h->ktime[q];
   return 0;
 }
 

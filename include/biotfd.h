@@ -123,15 +123,18 @@ int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float 
 /* Informational. */
 const char *bq_last_error(bq_handle *h);
 int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
-/* Average device time per step of each kernel of the last bq_run, ms, from
- * HIP events on the handle's stream: out[0] k_stress, out[1] k_velocity,
- * out[2] k_edges, out[3] k_sample. Timing must have been enabled with
- * bq_enable_timing(h, 1) before that run. */
+/* Average device time per step of each launch of the last bq_run, ms, from
+ * HIP events on the handle's stream: out[0] k_step (both half-steps of the
+ * interior cells, the stress half-step of all other cells and the previous
+ * step's receiver samples), out[1] k_velocity (velocity half-step of the other
+ * cells), out[2] k_edges (rows 1 and PMLZH). Timing must have been enabled
+ * with bq_enable_timing(h, 1) before that run. */
 int bq_enable_timing(bq_handle *h, int on);
-int bq_kernel_times(bq_handle *h, double out[4]);
-/* Cells updated per launch: out[0] k_stress, out[1] k_velocity, and of
- * those, the cells of the interior runs: out[2] stress, out[3] velocity. */
-int bq_kernel_cells(bq_handle *h, long long out[4]);
+int bq_kernel_times(bq_handle *h, double out[3]);
+/* Cells updated per step: out[0] by both half-steps inside k_step (fused
+ * blocks), out[1] by the stress half-step of k_step outside them, out[2] by
+ * k_velocity. */
+int bq_kernel_cells(bq_handle *h, long long out[3]);
 void bq_destroy(bq_handle *h);
 
 /* Diagnostic with no counterpart in the reference: streams nfloats binary32

@@ -55,3 +55,17 @@ def medium_deep(steps=200, tpml=12, mx=230, mz=140, f0=45.0) -> Job:
 
 
 GPU_EXTRA = {"medium": medium_layered, "deep": medium_deep}
+
+
+def wide_deep(steps=160, tpml=9, mx=600, mz=170, f0=30.0) -> Job:
+    """Wide enough for the fused interior blocks (256 columns); correlated random
+    model. The source cell (279, 68) lies in the last column and row owned by a
+    fused block, so the neighbouring blocks recompute it in their halos."""
+    job = models.layered_job(name="WIDE", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=4, seed=13,
+                             tpml=tpml, f0=f0, n_rec=50, field_std=0.08, corr_len=16.0, levels=5)
+    job.x_source = 279.0 * 2.0
+    job.z_source = 66.0 * 2.0
+    return job
+
+
+GPU_EXTRA["wide"] = wide_deep
