@@ -43,7 +43,7 @@ def make_workload(steps: int, mx=8000, mz=4000):
     return job
 
 
-def cpu_reference_baseline(job_full, steps_a=2, steps_b=7, budget_s=240.0):
+def cpu_reference_baseline(job_full, steps_a=2, steps_b=12, budget_s=300.0):
     """Times the reference (built from its sources into oracle/_ref/fd2d, serial,
     one core) on the same grid: per-step time from two runs that differ only in
     the step count, so set-up and input reading cancel."""
@@ -141,7 +141,7 @@ def main():
             cpu = {"value": round(cells / per_step / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1,
                    "kind": "reference",
                    "sample": f"reference fd2d (flang -O2 -ffp-contract=off, serial) on the same "
-                             f"{nx}x{nz} grid, 7 minus 2 steps, no output"}
+                             f"{nx}x{nz} grid, 12 minus 2 steps, no output"}
 
     out = {
         "metric": "Gcell-updates/s & %HBM roofline",
