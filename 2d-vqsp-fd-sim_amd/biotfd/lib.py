@@ -74,7 +74,10 @@ def load(path: str = LIBPATH):
         "bq_kernel_cells": ([p, p], C.c_int),
     }
     for name, (args, res) in sig.items():
-        f = getattr(lib, name)
+        try:
+            f = getattr(lib, name)
+        except AttributeError:
+            raise BiotError(f"{path} does not export {name}") from None
         f.argtypes = args
         f.restype = res
     _lib = lib

@@ -1053,22 +1053,33 @@ static KParams step_params(const bq_handle *h) {
   return k;
 }
 
+// Message of the last failed bq_create on this thread (there is no handle to
+// hold it); bq_last_error(NULL) returns it.
+static thread_local std::string g_create_err;
+
+static int create_fail(int rc, const std::string &msg) {
+  g_create_err = "bq_create: " + msg;
+  return rc;
+}
+
 extern "C" {
 
 int bq_create(const bq_desc *d, bq_handle **out) {
-  if (!d || !out) return -1;
+  if (!d || !out) return create_fail(-1, "null descriptor or output pointer");
   *out = nullptr;
+  g_create_err.clear();
   if (d->mx < 8 || d->mz < 6 || d->tpml < 1 || d->tpml > 256 || d->jmnum < 1 || d->mt2 < 1)
-    return -2;
-  if (!d->stress_imaging) return -3;
+    return create_fail(-2, "need mx >= 8, mz >= 6, 1 <= tpml <= 256, jmnum >= 1, mt2 >= 1");
+  if (!d->stress_imaging)
+    return create_fail(-3, "stress_imaging = 0 (explicit free surface) is not supported");
   if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
       d->lin_x > d->mx - 1 + d->tpml)
-    return -4;
-  if (d->reserved != 0) return -1;
+    return create_fail(-4, "source cell outside lin_x in [2-tpml, mx-1+tpml], lin_z in [2, mz-2]");
+  if (d->reserved != 0) return create_fail(-1, "reserved must be 0");
   const long long nx = (long long)d->mx + 2LL * d->tpml;
   const long long nz = (long long)d->mz + d->tpml + 1;
   const long long pitch = (nx + 63) / 64 * 64;
-  if (pitch * nz >= (1LL << 31)) return -2;
+  if (pitch * nz >= (1LL << 31)) return create_fail(-2, "grid of 2^31 cells or more per field");
   bq_handle *h = new bq_handle();
   GridDesc &g = h->g;
   g.MX = d->mx; g.MZ = d->mz; g.T = d->tpml; g.PMLL = 1 - g.T; g.PMLXH = g.MX + g.T;
@@ -1078,7 +1089,10 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   h->P = (int)pitch;
   (void)hipGetDevice(&h->device);
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete h; return -100; }
+  if (e != hipSuccess) {
+    delete h;
+    return create_fail(-100, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+  }
   KParams &k = h->kp;
   k.MX = g.MX; k.MZ = g.MZ; k.T = g.T; k.PMLL = g.PMLL; k.PMLZH = g.PMLZH; k.NX = g.NX;
   k.P = h->P; k.OFF = -g.PMLL; k.LIN_X = g.LIN_X; k.LIN_Z = g.LIN_Z; k.MR = 0;
@@ -1095,7 +1109,11 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   const size_t fsz = (size_t)h->P * NZ;
   rc |= dalloc(h, &h->fields[0], 8 * fsz);
   rc |= dalloc(h, &h->fields[1], 8 * fsz);
-  if (rc) { bq_destroy(h); return rc; }
+  if (rc) {
+    create_fail(rc, h->err);
+    bq_destroy(h);
+    return rc;
+  }
   const float **f0[8] = {&k.U0, &k.W0, &k.QX0, &k.QZ0, &k.TXX0, &k.TZZ0, &k.TXZ0, &k.PR0};
   float **f1[8] = {&k.U1, &k.W1, &k.QX1, &k.QZ1, &k.TXX1, &k.TZZ1, &k.TXZ1, &k.PR1};
   for (int f = 0; f < 8; ++f) {
@@ -1146,8 +1164,16 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   float2 *ds = nullptr;
   rc |= dalloc(h, &ds, (size_t)g.MT2);
   k.dsrc = ds;
-  if (rc) { int r = rc; bq_destroy(h); return r < 0 ? r : -101; }
-  if (hipStreamSynchronize(h->stream) != hipSuccess) { bq_destroy(h); return -100; }
+  if (rc) {
+    create_fail(-101, h->err);
+    bq_destroy(h);
+    return -101;
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    create_fail(-100, "stream synchronize after clearing the grid failed");
+    bq_destroy(h);
+    return -100;
+  }
   // Interior tiles start at the first column >= 5 whose address is 128-byte
   // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
   int fx = 5;
@@ -1200,7 +1226,10 @@ void bq_destroy(bq_handle *h) {
   delete h;
 }
 
-const char *bq_last_error(bq_handle *h) { return h ? h->err.c_str() : "null handle"; }
+const char *bq_last_error(bq_handle *h) {
+  if (h) return h->err.c_str();
+  return g_create_err.empty() ? "null handle" : g_create_err.c_str();
+}
 
 int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz) {
   if (!h) return -1;

@@ -51,7 +51,7 @@ def cpu_reference_baseline(job_full, steps_a=2, steps_b=12, budget_s=300.0):
     if not os.path.exists(exe):
         return None
     times = {}
-    tmp = tempfile.mkdtemp(prefix="bqref_", dir=os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None)
+    tmp = tempfile.mkdtemp(prefix="bqref_")
     try:
         for n in (steps_a, steps_b):
             job = make_like(job_full, n)

@@ -7,7 +7,8 @@
  * This ABI replaces the body of the step loop (time_loop.F90:91-190) with a
  * handle that owns the device-resident grid. A Fortran TIME_LOOP replacement
  * binds these entry points with ISO_C_BINDING (scalars by VALUE, arrays as
- * assumed-size dummies); see INTEGRATION.md for the interface block.
+ * assumed-size dummies); the interface block is MODULE BQ_BINDING in
+ * 2d-vqsp-fd-sim_amd/fortran/bq_binding.f90.
  *
  * Conventions
  *  - Fortran (x,z) arrays are C [z][x] arrays with the lower bounds removed:

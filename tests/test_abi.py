@@ -50,5 +50,9 @@ def test_create_rejects_bad_descriptions():
         lib.Desc(40, 30, 8, 1, 2, 29, 0, 10, 1, 0, 1.0, 1e-4),   # source row past MZ'-2
         lib.Desc(40, 30, 8, 1, 60, 5, 0, 10, 1, 0, 1.0, 1e-4),   # source column outside
     ]
-    for d in bad:
+    words = ["mx >= 8", "tpml", "stress_imaging", "source cell", "source cell"]
+    for d, w in zip(bad, words):
         assert so.bq_create(C.byref(d), C.byref(h)) < 0
+        assert h.value is None
+        msg = so.bq_last_error(None).decode()
+        assert msg.startswith("bq_create: ") and w in msg, msg
