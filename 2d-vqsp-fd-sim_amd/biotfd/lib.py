@@ -14,7 +14,8 @@ from . import hostsetup
 from .formats import Job
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libbiotfd.so")
+# the in-tree build; BIOTFD_LIB names another build of the same library
+LIBPATH = os.environ.get("BIOTFD_LIB") or os.path.join(HERE, "libbiotfd.so")
 
 FIELDS = ("U", "W", "QX", "QZ", "TXX", "TZZ", "TXZ", "P")
 EXPORTS = (
@@ -22,6 +23,7 @@ EXPORTS = (
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells", "bq_probe_copy",
+    "bq_probe_march",
 )
 
 
@@ -72,6 +74,7 @@ def load(path: str = LIBPATH):
         "bq_kernel_times": ([p, p], C.c_int),
         "bq_probe_copy": ([C.c_longlong, C.c_int, p], C.c_int),
         "bq_kernel_cells": ([p, p], C.c_int),
+        "bq_probe_march": ([C.c_int, C.c_int, C.c_int, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -82,6 +85,16 @@ def load(path: str = LIBPATH):
         f.restype = res
     _lib = lib
     return lib
+
+
+def probe_march(nx: int, nz: int, iters: int = 20) -> float:
+    """Read+write GB/s of the fused tiles' row-streaming access pattern (nine
+    arrays read, eight written, no arithmetic) on an nx x nz grid."""
+    out = np.zeros(1, dtype=np.float64)
+    rc = load().bq_probe_march(int(nx), int(nz), int(iters), out.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise BiotError(f"bq_probe_march failed ({rc})")
+    return float(out[0])
 
 
 def probe_copy(nfloats: int, iters: int = 20) -> float:
@@ -163,8 +176,10 @@ class Simulation:
     def sync(self):
         self._chk(self.lib.bq_sync(self.h))
 
-    def enable_timing(self, on: bool = True):
-        self._chk(self.lib.bq_enable_timing(self.h, int(on)))
+    def enable_timing(self, every: int = 1):
+        """Events around the launches of every ``every``-th step of each run
+        (0: off)."""
+        self._chk(self.lib.bq_enable_timing(self.h, int(every)))
 
     def kernel_times(self) -> dict:
         """ms per step of each launch: k_step (fused interior + boundary stress +

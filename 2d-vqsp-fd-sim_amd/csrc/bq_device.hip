@@ -31,7 +31,11 @@
 #include "../../include/biotfd.h"
 #include "bq_internal.h"
 
+using bq::Fused;
 using bq::GridDesc;
+using bq::Split;
+using bq::kMaxCells;
+using bq::kMaxRuns;
 
 namespace {
 
@@ -373,15 +377,6 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
 // cross-lane shuffles, with the 2 columns on either side of the 64 loaded by
 // four halo lanes. The arithmetic is the same expression as the per-cell path.
 
-struct Split {
-  // interior runs: up to 3 rectangles of 64-column strips, [rx0 + 64*ntx) x [rz0, rz1),
-  // each in blocks of 64 columns x 4 runs of nr rows (rblk blocks, rcum cumulative)
-  int nr, nrect;
-  int qx0[3], qntx[3], qz0[3], qz1[3], qblk[3], qcum[4];
-  int rx0[4], rx1[4], rz0[4], rz1[4];      // remaining cells: four rectangles [x0,x1) x [z0,z1)
-  int cum[5];                              // cumulative cell counts of the rectangles
-};
-
 // Values of a row at columns i-2, i-1, i+1, i+2 for lane i of a 64-lane tile
 // starting at column c; h holds the halo: lane 0 -> c+64, 1 -> c+65, 2 -> c-1, 3 -> c-2.
 struct XNb {
@@ -552,7 +547,7 @@ constexpr int kRun = 4;      // rows per wavefront run
 constexpr int kFR = 128;     // rows per fused block
 
 __device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l) {
-  if (g >= sp.cum[4]) return false;
+  if (g >= sp.cum[kMaxCells]) return false;
   int r = 0;
   while (g >= sp.cum[r + 1]) ++r;
   const int off = g - sp.cum[r];
@@ -598,13 +593,10 @@ __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nr
 // rings; the block's own column keeps its z-windows in registers. Fields are
 // read from the current set and written to the next, so one step moves
 // 8 fields in, 8 fields out and one material index per cell.
-constexpr int kFW = 256;       // columns per block, one thread each
+constexpr int kFW = 512;       // columns per block, one thread each
 constexpr int kLW = kFW + 8;   // old-velocity LDS row: columns c-4 .. c+kFW+3
 constexpr int kSW = kFW + 4;   // new-stress LDS row: columns c-2 .. c+kFW+1
-
-struct Fused {
-  int vx0, nfx, vz0, vz1, nfz;  // blocks: columns vx0 + kFW*bx, rows vz0 + NR*bz (clipped to vz1)
-};
+constexpr int kStepBlock = kFW;  // threads of a k_step block
 
 // New TXX, TZZ, TXZ, PRES at LDS column q (global column i) of row s, from the
 // old-velocity rings (same expressions as stress_run).
@@ -816,14 +808,16 @@ struct Launch1 {
 };
 
 template <int NR, int FR>
-__global__ __launch_bounds__(kBlock) void k_step(KParams k, Fused fu, Split sp, Launch1 ln,
-                                                 int itile) {
+__global__ __launch_bounds__(kStepBlock) void k_step(KParams k, Fused fu, Split sp, Launch1 ln,
+                                                     int itile) {
   __shared__ FusedLds L;
   int b = blockIdx.x;
   if (b < ln.nfused) {
     fused_block<FR>(k, fu, xcd_tile(b, ln.nfused), itile, L);
     return;
   }
+  // the other parts work in kBlock threads; the rest of the block has no work
+  if (threadIdx.x >= kBlock) return;
   b -= ln.nfused;
   if (b < ln.nrest) {
     int i, l;
@@ -939,6 +933,34 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
   }
 }
 
+// Row-streaming probe: blocks of kFW columns x kFR rows march down nine input
+// arrays and write eight output arrays two rows behind, with no stencil work -
+// the access pattern of the fused tiles without their arithmetic. Columns
+// [0, nbx*kFW), rows [4, 4 + nbz*kFR + 4) of [nz][pitch] arrays.
+struct MarchArrays {
+  const float *in[9];
+  float *out[8];
+};
+
+__global__ __launch_bounds__(kFW) void k_march_probe(MarchArrays A, int pitch, int nbx) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, bz = tile / nbx;
+  const int c = bx * kFW + threadIdx.x;
+  const int a = 4 + bz * kFR;
+  for (int s = a; s < a + kFR + 4; ++s) {
+    const size_t r = (size_t)s * pitch + c;
+    float v = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) v += A.in[f][r];
+    const int l = s - 2;
+    if (l >= a && l < a + kFR) {
+      const size_t rl = (size_t)l * pitch + c;
+#pragma unroll
+      for (int f = 0; f < 8; ++f) A.out[f][rl] = v;
+    }
+  }
+}
+
 }  // namespace
 
 // ===================================================================== host
@@ -957,7 +979,7 @@ struct bq_handle {
   bool model_ok = false, pml_ok = false, src_ok = false;
   int next_step = 1;
   int mr = 0;
-  bool timing = false;
+  int timing = 0;  // 0: off; n: events around the launches of every n-th step
   std::vector<hipEvent_t> ev;
   double ktime[3] = {0, 0, 0};
   int timed_steps = 0;
@@ -969,46 +991,6 @@ struct bq_handle {
 
 namespace {
 
-// Partition of the cells a half-step updates, x in [PMLL, PMLXH] and rows
-// [z_lo, z_hi]: the interior runs over the given rectangles (64-column strips
-// from x0 to x1, rows [z0, z1)) and per-cell blocks for every cell outside the
-// base rectangle [bx0, bx1) x [bz0, bz1) that encloses the runs and the fused
-// tiles. The runs read rows up to z1+2 and columns x0-2 .. x1+1; with
-// z1 <= MZ'-1, TPML >= 1 and 4 <= x0, x1 <= MX-1 these stay inside the grid.
-struct RunRect { int x0, x1, z0, z1; };
-
-Split make_split(const GridDesc &g, int z_lo, int z_hi, int bx0, int bx1, int bz0, int bz1,
-                 const RunRect *rr, int nrr, int nr) {
-  Split s{};
-  s.nr = nr;
-  s.nrect = 0;
-  s.qcum[0] = 0;
-  for (int q = 0; q < nrr; ++q) {
-    const int ntx = rr[q].x1 > rr[q].x0 ? (rr[q].x1 - rr[q].x0) / 64 : 0;
-    const int nz = rr[q].z1 > rr[q].z0 ? (rr[q].z1 - rr[q].z0 + 4 * nr - 1) / (4 * nr) : 0;
-    if (ntx * nz == 0) continue;
-    const int k = s.nrect++;
-    s.qx0[k] = rr[q].x0; s.qntx[k] = ntx; s.qz0[k] = rr[q].z0; s.qz1[k] = rr[q].z1;
-    s.qblk[k] = ntx * nz;
-    s.qcum[k + 1] = s.qcum[k] + s.qblk[k];
-  }
-  for (int k = s.nrect; k < 3; ++k) s.qcum[k + 1] = s.qcum[k];
-  if (bx1 <= bx0 || bz1 <= bz0) {  // everything on the per-cell path
-    s.rx0[0] = g.PMLL; s.rx1[0] = g.PMLXH + 1; s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
-    for (int r = 1; r < 4; ++r) { s.rx0[r] = s.rx1[r] = 0; s.rz0[r] = s.rz1[r] = 0; }
-  } else {
-    s.rx0[0] = g.PMLL; s.rx1[0] = bx0;         s.rz0[0] = z_lo; s.rz1[0] = z_hi + 1;
-    s.rx0[1] = bx1;    s.rx1[1] = g.PMLXH + 1; s.rz0[1] = z_lo; s.rz1[1] = z_hi + 1;
-    s.rx0[2] = bx0;    s.rx1[2] = bx1;         s.rz0[2] = z_lo; s.rz1[2] = bz0;
-    s.rx0[3] = bx0;    s.rx1[3] = bx1;         s.rz0[3] = bz1;  s.rz1[3] = z_hi + 1;
-  }
-  s.cum[0] = 0;
-  for (int r = 0; r < 4; ++r) {
-    const int w = s.rx1[r] - s.rx0[r], hgt = s.rz1[r] - s.rz0[r];
-    s.cum[r + 1] = s.cum[r] + (w > 0 && hgt > 0 ? w * hgt : 0);
-  }
-  return s;
-}
 
 int fail(bq_handle *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -1174,45 +1156,15 @@ int bq_create(const bq_desc *d, bq_handle **out) {
     bq_destroy(h);
     return -100;
   }
-  // Interior tiles start at the first column >= 5 whose address is 128-byte
-  // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
-  int fx = 5;
-  while ((fx + k.OFF) % 32 != 0) ++fx;
-  const int ntx = g.MX - 2 >= fx ? (g.MX - 2 - fx + 1) / 64 : 0;
-  const int fxe = fx + 64 * ntx;  // runs cover columns [fx, fxe), inside [5, MX-2]
-  // Fused tiles: columns [fx, vx1), vx1 <= MX-3, rows [5, MZ'-3): their new
-  // stresses reach two rows and columns further, still inside [3, MX-2] x
-  // [3, MZ'-2], where both half-steps use the interior formulas.
-  Fused &fu = h->fu;
-  fu.vx0 = fx;
-  fu.nfx = g.MX - 3 > fx ? (g.MX - 3 - fx) / kFW : 0;
-  fu.vz0 = 5;
-  fu.vz1 = g.MZ - 3;
-  fu.nfz = (fu.nfx > 0 && fu.vz1 > fu.vz0) ? (fu.vz1 - fu.vz0 + kFR - 1) / kFR : 0;
-  if (fu.nfz == 0) fu.nfx = 0;
-  const int vx1 = fu.nfx ? fx + kFW * fu.nfx : fx;
-  h->fused_cells = (long long)kFW * fu.nfx * (fu.vz1 - fu.vz0) * (fu.nfz ? 1 : 0);
-  // stress half-step outside the fused tiles: runs on [vx1, fxe) x [5, MZ'-1)
-  // and [fx, vx1) x [MZ'-3, MZ'-1); rows 2-4 (image rows), the sides and the
-  // bottom rows on the per-cell path
-  if (fu.nfx) {
-    const RunRect rs[2] = {{vx1, fxe, 5, g.MZ - 1}, {fx, vx1, fu.vz1, g.MZ - 1}};
-    h->ss = make_split(g, 2, g.PMLZH, fx, fxe, 5, g.MZ - 1, rs, 2, kRun);
-    const RunRect rv[3] = {{vx1, fxe, 2, g.MZ - 1}, {fx, vx1, 2, 5}, {fx, vx1, fu.vz1, g.MZ - 1}};
-    h->sv = make_split(g, 2, g.PMLZH - 1, fx, fxe, 2, g.MZ - 1, rv, 3, kRun);
-  } else {
-    const RunRect rs[1] = {{fx, fxe, 5, g.MZ - 1}};
-    h->ss = make_split(g, 2, g.PMLZH, fx, fxe, 5, g.MZ - 1, rs, 1, kRun);
-    const RunRect rv[1] = {{fx, fxe, 2, g.MZ - 1}};
-    h->sv = make_split(g, 2, g.PMLZH - 1, fx, fxe, 2, g.MZ - 1, rv, 1, kRun);
-  }
-  auto runs_cells = [](const Split &sp) {
-    long long n = sp.cum[4];
-    for (int q = 0; q < sp.nrect; ++q) n += 64LL * sp.qntx[q] * (sp.qz1[q] - sp.qz0[q]);
-    return n;
-  };
-  h->stress_cells = runs_cells(h->ss);
-  h->velocity_cells = runs_cells(h->sv);
+  // cells of the stress half-step from row 2 (rows 0, 1 are its images), of
+  // the velocity half-step from row 2 (row 1 is k_edges')
+  const bq::Plan plan = bq::plan_cells(g, k.OFF, kFW, kFR, kRun, 2, 2);
+  h->fu = plan.fu;
+  h->ss = plan.ss;
+  h->sv = plan.sv;
+  h->fused_cells = plan.fused_cells;
+  h->stress_cells = plan.stress_cells;
+  h->velocity_cells = plan.velocity_cells;
   *out = h;
   return 0;
 }
@@ -1394,9 +1346,10 @@ int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *
   return 0;
 }
 
-int bq_enable_timing(bq_handle *h, int on) {
+int bq_enable_timing(bq_handle *h, int every) {
   if (!h) return -1;
-  h->timing = on != 0;
+  if (every < 0) return fail(h, -2, "bq_enable_timing: negative step stride");
+  h->timing = every;
   return 0;
 }
 
@@ -1411,16 +1364,18 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   const dim3 blk(kBlock);
   Launch1 ln{};
   ln.nfused = h->fu.nfx * h->fu.nfz;
-  ln.nrest = (h->ss.cum[4] + kBlock - 1) / kBlock;
-  ln.nruns = h->ss.qcum[3];
-  const int vrest = (h->sv.cum[4] + kBlock - 1) / kBlock;
-  const int vgrid = vrest + h->sv.qcum[3];
+  ln.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
+  ln.nruns = h->ss.qcum[kMaxRuns];
+  const int vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;
+  const int vgrid = vrest + h->sv.qcum[kMaxRuns];
   const int nsb = (h->mr + kBlock - 1) / kBlock;
   const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
   const int nsteps = itile_end - itile_begin + 1;
   constexpr int kEv = 4;
-  if (h->timing) {
-    const size_t need = (size_t)nsteps * kEv;
+  const int every = h->timing;
+  const int ntimed = every ? (nsteps + every - 1) / every : 0;  // steps 0, every, 2*every, ...
+  if (every) {
+    const size_t need = (size_t)ntimed * kEv;
     while (h->ev.size() < need) {
       hipEvent_t e;
       HIPCHK(h, hipEventCreate(&e));
@@ -1434,9 +1389,12 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
     ln.sample_step = it - 1;
     const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
-    hipEvent_t *e = h->timing ? &h->ev[(size_t)(it - itile_begin) * kEv] : nullptr;
+    const int rel = it - itile_begin;
+    hipEvent_t *e = (every && rel % every == 0) ? &h->ev[(size_t)(rel / every) * kEv] : nullptr;
     if (e) (void)hipEventRecord(e[0], h->stream);
-    if (g1) hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), blk, 0, h->stream, k, h->fu, h->ss, ln, it);
+    if (g1)
+      hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu,
+                         h->ss, ln, it);
     if (e) (void)hipEventRecord(e[1], h->stream);
     if (vgrid) hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
     if (e) (void)hipEventRecord(e[2], h->stream);
@@ -1447,18 +1405,18 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   }
   HIPCHK(h, hipGetLastError());
   h->next_step = itile_end + 1;
-  if (h->timing) {
+  if (every) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     double acc[kEv - 1] = {0, 0, 0};
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = 0; s < ntimed; ++s) {
       for (int q = 0; q < kEv - 1; ++q) {
         float ms = 0;
         HIPCHK(h, hipEventElapsedTime(&ms, h->ev[(size_t)s * kEv + q], h->ev[(size_t)s * kEv + q + 1]));
         acc[q] += ms;
       }
     }
-    for (int q = 0; q < kEv - 1; ++q) h->ktime[q] = acc[q] / nsteps;
-    h->timed_steps = nsteps;
+    for (int q = 0; q < kEv - 1; ++q) h->ktime[q] = acc[q] / ntimed;
+    h->timed_steps = ntimed;
   }
   return 0;
 }
@@ -1502,6 +1460,45 @@ int bq_probe_copy(long long nfloats, int iters, double *gbps) {
   (void)hipFree(a);
   (void)hipFree(b);
   return 0;
+}
+
+int bq_probe_march(int nx, int nz, int iters, double *gbps) {
+  if (nx < kFW || nz < kFR + 12 || iters < 1 || !gbps) return -1;
+  const int pitch = (nx + 63) / 64 * 64;
+  const int nbx = nx / kFW, nbz = (nz - 8) / (kFR + 4);
+  // rows touched: 4 .. 4 + nbz*kFR + 3 < nz; columns < nbx*kFW <= nx <= pitch
+  const size_t n = (size_t)pitch * nz;
+  std::vector<float *> buf(17, nullptr);
+  int rc = 0;
+  for (auto &p : buf) {
+    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { p = nullptr; rc = -101; break; }
+    (void)hipMemset(p, 0, n * sizeof(float));
+  }
+  if (rc == 0) {
+    MarchArrays A{};
+    for (int f = 0; f < 9; ++f) A.in[f] = buf[f];
+    for (int f = 0; f < 8; ++f) A.out[f] = buf[9 + f];
+    const int blocks = nbx * nbz;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < iters; ++r)
+      hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double cells = (double)nbx * kFW * nbz * kFR;
+    *gbps = cells * 17.0 * sizeof(float) * iters / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (hipGetLastError() != hipSuccess) rc = -100;
+  }
+  for (auto p : buf)
+    if (p) (void)hipFree(p);
+  return rc;
 }
 
 int bq_sync(bq_handle *h) {

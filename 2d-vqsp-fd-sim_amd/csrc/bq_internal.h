@@ -31,6 +31,48 @@ struct PmlHost {
   }
 };
 
+// ---- partition of a step's cells among the kernels
+
+// Cells [x0, x1) x [z0, z1).
+struct Rect {
+  int x0, x1, z0, z1;
+};
+
+constexpr int kMaxRuns = 3;   // run rectangles of one half-step
+constexpr int kMaxCells = 6;  // per-cell rectangles of one half-step
+
+// The cells of one half-step outside the fused tiles. Runs: up to kMaxRuns
+// rectangles of 64-column strips [qx0, qx0 + 64*qntx) x [qz0, qz1), in blocks
+// of 64 columns x 4 runs of nr rows (qblk blocks, qcum cumulative). Per-cell
+// path: up to kMaxCells rectangles [rx0, rx1) x [rz0, rz1), one thread per
+// cell, cum the cumulative cell counts.
+struct Split {
+  int nr, nrect;
+  int qx0[kMaxRuns], qntx[kMaxRuns], qz0[kMaxRuns], qz1[kMaxRuns], qblk[kMaxRuns];
+  int qcum[kMaxRuns + 1];
+  int rx0[kMaxCells], rx1[kMaxCells], rz0[kMaxCells], rz1[kMaxCells];
+  int cum[kMaxCells + 1];
+};
+
+// Fused tiles: columns [vx0, vxe) in nfx tiles of the tile width, rows
+// [vz0, vz1) in nfz blocks of the block height (the last one clipped).
+// nfx = 0: no fused tiles.
+struct Fused {
+  int vx0, vxe, nfx, vz0, vz1, nfz;
+};
+
+struct Plan {
+  Fused fu;
+  Split ss, sv;  // stress and velocity half-step outside the fused tiles
+  long long fused_cells, stress_cells, velocity_cells;
+};
+
+// Partition of the cells of one step: the stress half-step covers x in
+// [PMLL, PMLXH], rows [zs, PMLZH]; the velocity half-step rows [zv, PMLZH-1]
+// (the rows outside these are the kernels' own edge rows). off = -PMLL; fw,
+// fr: fused tile width and height; nr: rows per wavefront run.
+Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv);
+
 void wave_speeds(float sig1, float p, float ainv, float mu, float rel1, float rel2, float rel3,
                  float *vpf, float *vps, float *vs);
 void build_pml(const GridDesc &g, const int32_t *jm, const float *tab, float r, float tau,

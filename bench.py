@@ -34,6 +34,7 @@ HBM_PEAK_GBS = 8000.0
 B_STEP = 104          # algorithmic bytes per cell per step, two half-steps (indexed materials)
 B_FUSED = 68          # fused cell: 8 fields read, 8 fields written, 1 int32 index
 B_HALF = 52           # one half-step: 4 fields read, 4 read and written, 1 int32 index
+TIMING_STRIDE = 8     # kernel durations from HIP events around every 8th step
 
 
 def make_workload(steps: int, mx=8000, mz=4000):
@@ -90,7 +91,8 @@ def main():
     ap.add_argument("--mz", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="also time a plain 1 GiB device copy (k_copy_probe)")
+                    help="also time a plain 1 GiB device copy (k_copy_probe) and the fused "
+                         "tiles' access pattern without arithmetic (k_march_probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -106,7 +108,8 @@ def main():
     print("[bench] uploaded", file=sys.stderr, flush=True)
     sim.run(args.warmup)
     print("[bench] warmup done", file=sys.stderr, flush=True)
-    sim.enable_timing(True)
+    # HIP events around the launches of every 8th step of the timed region
+    sim.enable_timing(TIMING_STRIDE)
     sim.sync()
     t1 = time.perf_counter()
     sim.run(args.steps, sync=False)
@@ -130,9 +133,10 @@ def main():
         with open(tr_path) as f:
             traffic = json.load(f).get(dom_name)
 
-    copy_gbs = None
+    copy_gbs = march_gbs = None
     if args.copy_probe:
         copy_gbs = lib.probe_copy(1 << 28, 20)
+        march_gbs = lib.probe_march(nx, nz, 20)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -166,6 +170,7 @@ def main():
     }
     if copy_gbs is not None:
         out["copy_probe_gbs"] = round(copy_gbs, 1)
+        out["march_probe_gbs"] = round(march_gbs, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -129,8 +129,10 @@ int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
  * interior cells, the stress half-step of all other cells and the previous
  * step's receiver samples), out[1] k_velocity (velocity half-step of the other
  * cells), out[2] k_edges (rows 1 and PMLZH). Timing must have been enabled
- * with bq_enable_timing(h, 1) before that run. */
-int bq_enable_timing(bq_handle *h, int on);
+ * with bq_enable_timing before that run: every = n >= 1 brackets the launches
+ * of every n-th step of a run (its first step included) with events, 0 turns
+ * timing off. */
+int bq_enable_timing(bq_handle *h, int every);
 int bq_kernel_times(bq_handle *h, double out[3]);
 /* Cells updated per step: out[0] by both half-steps inside k_step (fused
  * blocks), out[1] by the stress half-step of k_step outside them, out[2] by
@@ -143,6 +145,13 @@ void bq_destroy(bq_handle *h);
  * and reports read+write GB/s. Used to measure the achievable HBM bandwidth
  * and to calibrate hardware traffic counters. */
 int bq_probe_copy(long long nfloats, int iters, double *gbps);
+
+/* Diagnostic with no counterpart in the reference: the access pattern of the
+ * fused step tiles without their arithmetic (kernel k_march_probe). Blocks of
+ * 512 columns x 128 rows march down nine [nz][nx] arrays and write eight, one
+ * row at a time; reports read+write GB/s, the practical bandwidth ceiling of
+ * that row-streaming pattern. Needs nx >= 512, nz >= 140. */
+int bq_probe_march(int nx, int nz, int iters, double *gbps);
 
 #ifdef __cplusplus
 }

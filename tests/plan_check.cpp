@@ -1,0 +1,90 @@
+// Exhaustive check of the cell partition (bq::plan_cells) on a list of grids:
+// every cell of each half-step is updated by exactly one of the fused tiles,
+// the runs and the per-cell path, and every kernel's reads stay in the grid.
+// Usage: plan_check MX MZ' TPML [MX MZ' TPML ...]; prints "ok <grids>".
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "bq_internal.h"
+
+using namespace bq;
+
+static int fail(const char *what, int mx, int mz, int t, int x, int z) {
+  std::printf("FAIL %s grid %d x %d T=%d at x=%d z=%d\n", what, mx, mz, t, x, z);
+  return 1;
+}
+
+static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) {
+  GridDesc g{};
+  g.MX = MX; g.MZ = MZ; g.T = T; g.PMLL = 1 - T; g.PMLXH = MX + T; g.PMLZH = MZ + T;
+  g.NX = MX + 2 * T; g.NZ = MZ + T + 1;
+  const int off = T - 1;
+  const Plan p = plan_cells(g, off, fw, fr, nr, zs, zv);
+  const int NX = g.NX, NZ = g.NZ;
+  std::vector<int> cs((size_t)NX * NZ, 0), cv((size_t)NX * NZ, 0);
+  auto at = [&](std::vector<int> &v, int x, int z) -> int & { return v[(size_t)z * NX + (x - g.PMLL)]; };
+  auto inside = [&](int x, int z) { return x >= g.PMLL && x <= g.PMLXH && z >= 0 && z <= g.PMLZH; };
+  // fused tiles
+  const Fused &f = p.fu;
+  long long nf = 0;
+  for (int bz = 0; bz < f.nfz; ++bz)
+    for (int bx = 0; bx < f.nfx; ++bx) {
+      const int c = f.vx0 + bx * fw, a = f.vz0 + bz * fr;
+      const int e = std::min(a + fr, f.vz1), wd = fw;
+      if (e <= a || c + wd > f.vxe) return fail("empty fused tile", MX, MZ, T, c, a);
+      if (c < 5 || c + wd > MX - 3 || a < 5 || e > MZ - 3) return fail("fused tile range", MX, MZ, T, c, a);
+      // reads: columns c-4 .. c+fw+3, rows a-4 .. e+3
+      if (!inside(c - 4, a - 4) || !inside(c + fw + 3, e + 3)) return fail("fused reads", MX, MZ, T, c, a);
+      for (int z = a; z < e; ++z)
+        for (int x = c; x < c + wd; ++x) { ++at(cs, x, z); ++at(cv, x, z); ++nf; }
+    }
+  if (nf != p.fused_cells) return fail("fused count", MX, MZ, T, 0, 0);
+  auto runs = [&](const Split &s, std::vector<int> &cnt, int zmin, long long *n) -> int {
+    for (int q = 0; q < s.nrect; ++q) {
+      const int x0 = s.qx0[q], x1 = x0 + 64 * s.qntx[q];
+      if ((x0 + off) % 32 != 0) return fail("run alignment", MX, MZ, T, x0, s.qz0[q]);
+      if (x0 < 4 || x1 > MX - 1 || s.qz0[q] < zmin || s.qz1[q] > MZ - 1)
+        return fail("run range", MX, MZ, T, x0, s.qz0[q]);
+      const int nz = (s.qz1[q] - s.qz0[q] + 4 * s.nr - 1) / (4 * s.nr);
+      if (s.qblk[q] != s.qntx[q] * nz || s.qcum[q + 1] != s.qcum[q] + s.qblk[q])
+        return fail("run blocks", MX, MZ, T, x0, s.qz0[q]);
+      for (int z = s.qz0[q]; z < s.qz1[q]; ++z)
+        for (int x = x0; x < x1; ++x) { ++at(cnt, x, z); ++*n; }
+    }
+    for (int r = 0; r < kMaxCells; ++r)
+      for (int z = s.rz0[r]; z < s.rz1[r]; ++z)
+        for (int x = s.rx0[r]; x < s.rx1[r]; ++x) {
+          if (!inside(x, z)) return fail("cell outside grid", MX, MZ, T, x, z);
+          ++at(cnt, x, z);
+          ++*n;
+        }
+    return 0;
+  };
+  long long ns = 0, nv = 0;
+  if (runs(p.ss, cs, 5, &ns) || runs(p.sv, cv, 2, &nv)) return 1;
+  if (ns != p.stress_cells || nv != p.velocity_cells) return fail("cell counts", MX, MZ, T, 0, 0);
+  for (int z = 0; z <= g.PMLZH; ++z)
+    for (int x = g.PMLL; x <= g.PMLXH; ++x) {
+      const int want_s = z >= zs ? 1 : 0, want_v = (z >= zv && z <= g.PMLZH - 1) ? 1 : 0;
+      if (at(cs, x, z) != want_s) return fail("stress coverage", MX, MZ, T, x, z);
+      if (at(cv, x, z) != want_v) return fail("velocity coverage", MX, MZ, T, x, z);
+    }
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 9 || (argc - 6) % 3 != 0) {
+    std::printf("usage: plan_check FW FR NR ZS ZV MX MZ T [MX MZ T ...]\n");
+    return 2;
+  }
+  const int fw = std::atoi(argv[1]), fr = std::atoi(argv[2]), nr = std::atoi(argv[3]);
+  const int zs = std::atoi(argv[4]), zv = std::atoi(argv[5]);
+  int n = 0;
+  for (int a = 6; a + 2 < argc; a += 3) {
+    if (check(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), fw, fr, nr, zs, zv)) return 1;
+    ++n;
+  }
+  std::printf("ok %d\n", n);
+  return 0;
+}

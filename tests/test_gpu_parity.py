@@ -91,3 +91,22 @@ def test_bitexact_vs_reference_snapshots(name):
             got, ref = sim.field(fname), z["snaps"][k, f]
             bad = np.argwhere(got != ref)
             assert bad.size == 0, f"{name} step {step} {fname}: {len(bad)} cells differ"
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_kernel_timing_leaves_results_unchanged(every):
+    """Timed runs (events around every n-th step) give the untimed fields and
+    positive per-launch durations."""
+    job = jobs.tiny_layered()
+    print(f"timing every {every}", flush=True)
+    a, b = lib.Simulation(job), lib.Simulation(job)
+    a.enable_timing(every)
+    a.run(10)
+    b.run(10)
+    t = a.kernel_times()
+    assert set(t) == {"k_step", "k_velocity", "k_edges"}
+    assert all(v > 0 for v in t.values()), t
+    for f in FIELDS:
+        np.testing.assert_array_equal(a.field(f), b.field(f))
+    a.close()
+    b.close()

@@ -1,0 +1,40 @@
+"""Cell partition of a step (bq::plan_cells in the library's host code): every
+cell of each half-step is updated by exactly one of the fused tiles, the runs
+and the per-cell path, and every kernel's reads stay inside the grid. Runs on
+the CPU: the planner is plain C++ and is compiled here with g++."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "2d-vqsp-fd-sim_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("plan") / "plan_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", CSRC, os.path.join(ROOT, "tests", "plan_check.cpp"),
+                    os.path.join(CSRC, "bq_host.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def grids():
+    g = [(8000, 4002, 20), (4000, 2002, 20), (400, 402, 20), (600, 172, 9), (8, 6, 1), (40, 30, 8),
+         (300, 200, 20), (1100, 300, 5), (530, 140, 3), (520, 136, 2), (1038, 141, 30), (45, 12, 256)]
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        g.append((int(rng.integers(8, 1400)), int(rng.integers(6, 400)), int(rng.integers(1, 40))))
+    return g
+
+
+@pytest.mark.parametrize("fw,fr,nr", [(512, 128, 4), (256, 128, 4), (256, 32, 8)])
+@pytest.mark.parametrize("zs,zv", [(2, 2), (0, 0)])
+def test_partition_covers_each_cell_once(checker, fw, fr, nr, zs, zv):
+    args = [checker, str(fw), str(fr), str(nr), str(zs), str(zv)]
+    for mx, mz, t in grids():
+        args += [str(mx), str(mz), str(t)]
+    r = subprocess.run(args, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip() == f"ok {len(grids())}"
