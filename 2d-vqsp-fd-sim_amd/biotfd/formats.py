@@ -138,6 +138,9 @@ class Job:
     receivers: Sequence[tuple] = ()   # (irecx, lrecx, irecz, lrecz)
     tpml: int = 20
     stress_imaging: bool = True
+    # compile-time options of the reference's non-imaging free surface, as
+    # bits: 1 = AFDA_ZZ (strain_uw_zz_0.F90:52), 2 = AFDA_XZ (strain_uw_xz_1.F90:83)
+    afda: int = 0
     r: float = 0.001
     tau: float = 1.5
     pow: float = 2.0

@@ -22,7 +22,13 @@
  *
  * Parity status: pinned bit-exactly against snapshots of the reference built
  * from its own sources (oracle/Makefile -> oracle/_ref/fd2d), see
- * tests/golden/make_golden.py and tests/test_oracle_golden.py.
+ * tests/golden/make_golden.py and tests/test_oracle_golden.py. The surface
+ * rows without stress imaging (HPLANE_0/1_* non-imaging branches, with the
+ * AFDA_ZZ / AFDA_XZ options as run-time bits) follow the subroutine bodies;
+ * the reference itself faults in that variant (its INH_QZ_2ND_0 interface
+ * block does not match the subroutine), so parity against the reference is
+ * unpinned there and each surface stencil is pinned by polynomial exactness
+ * instead (tests/test_oracle_surface.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -52,14 +58,18 @@ enum {
 };
 
 typedef struct oracle_desc {
-  int32_t mx, mz, tpml, stress_imaging, lin_x, lin_z, ktbo, mt2, jmnum, reserved;
+  int32_t mx, mz, tpml, stress_imaging, lin_x, lin_z, ktbo, mt2, jmnum, variant;
   double h, dt;
 } oracle_desc;
+
+/* variant bits: the reference's compile-time options of the non-imaging
+ * free surface (strain_uw_zz_0.F90:52, strain_q_zz_0.F90:47, strain_uw_xz_1.F90:83) */
+enum { V_AFDA_ZZ = 1, V_AFDA_XZ = 2 };
 
 typedef struct {
   /* sizes (aux1.F90:36-46) */
   int MX, MZ, T, PMLL, PMLXH, PMLZH, NX, NZ, MX1;
-  int LIN_X, LIN_Z, SI, KTBO, MT2, JMNUM, next_step;
+  int LIN_X, LIN_Z, SI, AFDA_ZZ, AFDA_XZ, KTBO, MT2, JMNUM, next_step;
   real H, DT, A, B, AH, BH, CH;
   int32_t *JM;
   real *tab[T_N];
@@ -332,6 +342,75 @@ static void hplane_01_stress_si(orc *o) {
     PRF(I, 1) = -PRF(I, 2);
     TXZF(I, 2) = 0;
   }
+}
+
+/* STRAIN_UW_ZZ_0, strain_uw_zz_0.F90:45-62 (and STRAIN_Q_ZZ_0, strain_q_zz_0.F90:40-57):
+ * row 0 without stress imaging, one-sided 5-point form with AFDA_ZZ */
+static void strain_zz_0(orc *o, const real *F, real *E, real C) {
+  const int T = o->T, MXT1 = o->MX - 1;
+  for (int I = 2 - T; I <= MXT1 + T; ++I) RT(E, I) = (C * (F[IXF(o, I, 1)] - F[IXF(o, I, 0)]));
+  if (o->AFDA_ZZ)
+    for (int I = 2; I <= MXT1; ++I)
+      RT(E, I) = C * (-((real)11 / (real)12 * F[IXF(o, I, 0)]) + (real)17 / (real)24 * F[IXF(o, I, 1)] +
+                      (real)3 / (real)8 * F[IXF(o, I, 2)] - (real)5 / (real)24 * F[IXF(o, I, 3)] +
+                      (real)1 / (real)24 * F[IXF(o, I, 4)]);
+}
+
+/* STRAIN_UW_XZ_1, strain_uw_xz_1.F90:62-114: row 1 without stress imaging; the
+ * z-derivative of U is the 2-point one (U(i,0) is the surface row), or a
+ * one-sided 4-point one with AFDA_XZ */
+static void strain_uw_xz_1(orc *o, real *EXZ, real A, real B, real C) {
+  const int T = o->T, MX = o->MX, MXT1 = MX - 1, L = 1, L1 = 0;
+  for (int I = 2 - T; I <= 2; ++I) {
+    real DZXA = C * (WM(I, L) - WM(I - 1, L));
+    RT(EXZ, I) = C * (UM(I, L) - UM(I, L1)) +
+                 XL2(o->ODA2_XL, 3 - I, L) *
+                     (XL2(o->ODD2_XL, 3 - I, L) * DZXA + ML(o->PZXL, W_L2, I, L));
+    ML(o->PZXL, W_L2, I, L) =
+        XL2(o->ODB2_XL, 3 - I, L) * ML(o->PZXL, W_L2, I, L) + XL2(o->ODC2_XL, 3 - I, L) * DZXA;
+  }
+  for (int I = 3; I <= MX - 1; ++I) {
+    if (o->AFDA_XZ) {
+      real DUIKM1 = -(A * (WM(I + 1, 0) - WM(I - 2, 0))) - B * (WM(I, 0) - WM(I - 1, 0));
+      RT(EXZ, I) = A * (WM(I + 1, L) - WM(I - 2, L)) + B * (WM(I, L) - WM(I - 1, L)) -
+                   (real)1 / (real)22 * DUIKM1 +
+                   C * (-((real)577 / (real)528 * UM(I, 0)) + (real)201 / (real)176 * UM(I, 1) -
+                        (real)9 / (real)176 * UM(I, 2) + (real)1 / (real)528 * UM(I, 3));
+    } else {
+      RT(EXZ, I) = A * (WM(I + 1, L) - WM(I - 2, L)) + B * (WM(I, L) - WM(I - 1, L)) +
+                   C * (UM(I, 1) - UM(I, 0));
+    }
+  }
+  for (int I = MX; I <= MX + T; ++I) {
+    real DZXA = C * (WM(I, L) - WM(I - 1, L));
+    RT(EXZ, I) = C * (UM(I, L) - UM(I, L1)) +
+                 XL2(o->ODA2_XH, I - MXT1, L) *
+                     (XL2(o->ODD2_XH, I - MXT1, L) * DZXA + MR0(o->PZXH, W_R0, I, L));
+    MR0(o->PZXH, W_R0, I, L) = XL2(o->ODB2_XH, I - MXT1, L) * MR0(o->PZXH, W_R0, I, L) +
+                               XL2(o->ODC2_XH, I - MXT1, L) * DZXA;
+  }
+}
+
+/* HPLANE_0_STRESS without stress imaging, hplane_0_stress.F90:57-119: EXZ = 0 */
+static void hplane_0_stress(orc *o, rowtmp *t) {
+  for (int I = o->PMLL; I <= o->PMLXH; ++I) RT(t->EXZ, I) = 0;
+  strain_zz_0(o, o->f[F_W], t->EZZ, o->CH);
+  strain_zz_0(o, o->f[F_QZ], t->QZZ, o->CH);
+  strain_xx(o, 0, o->f[F_U], t->EXX, o->PXXL, o->PXXH, o->AH, o->BH, o->CH);
+  strain_xx(o, 0, o->f[F_QX], t->QXX, o->OXXL, o->OXXH, o->AH, o->BH, o->CH);
+  stress_xz(o, 0, t->EXZ);
+  stress_nn(o, 0, t->EXX, t->EZZ, t->QXX, t->QZZ, 0, 0, 0);
+}
+
+/* HPLANE_1_STRESS without stress imaging, hplane_1_stress.F90:51-126 */
+static void hplane_1_stress(orc *o, rowtmp *t) {
+  strain_uw_xz_1(o, t->EXZ, o->AH, o->BH, o->CH);
+  strain_zz(o, 1, o->f[F_W], t->EZZ, o->AH, o->BH, o->CH);
+  strain_zz(o, 1, o->f[F_QZ], t->QZZ, o->AH, o->BH, o->CH);
+  strain_xx(o, 1, o->f[F_U], t->EXX, o->PXXL, o->PXXH, o->AH, o->BH, o->CH);
+  strain_xx(o, 1, o->f[F_QX], t->QXX, o->OXXL, o->OXXH, o->AH, o->BH, o->CH);
+  stress_xz(o, 1, t->EXZ);
+  stress_nn(o, 1, t->EXX, t->EZZ, t->QXX, t->QZZ, 0, 0, 0);
 }
 
 /* ===================================================== velocity row kernels */
@@ -692,6 +771,194 @@ static void hplane_1_velocity_si(orc *o) {
   sto_qp(o, 1);
 }
 
+/* ---- rows 0 and 1 without stress imaging (hplane_0_velocity.F90:62-179,
+ *      hplane_1_velocity.F90:79-207) */
+
+/* lead + the z-derivative terms of U/QX on row 0 (TXZ(i,0) = 0 on the free
+ * surface), added term by term after lead as the reference's expression does
+ * (inh_qx_int_0.F90, inh_v_uv_int_0.F90) */
+#define TXZ_Z0(lead, I)                                                                        \
+  ((((lead) + (real)17 / (real)24 * TXZF(I, 1)) + (real)3 / (real)8 * TXZF(I, 2)) -             \
+   (real)5 / (real)24 * TXZF(I, 3) + (real)1 / (real)24 * TXZF(I, 4))
+/* one-sided z-derivatives on row 0 and row 1 (inh_qz_int_0.F90, inh_v_qz_int_1.F90) */
+#define DZ0(F, I)                                                                              \
+  ((real)35 / (real)8 * F(I, 0) - (real)35 / (real)24 * F(I, 1) + (real)21 / (real)40 * F(I, 2) - \
+   (real)5 / (real)56 * F(I, 3))
+#define DZ1(F, I)                                                                                \
+  (-((real)31 / (real)24 * F(I, 0)) + (real)29 / (real)24 * F(I, 1) - (real)3 / (real)40 * F(I, 2) + \
+   (real)1 / (real)168 * F(I, 3))
+/* lead - 31/24 F0 + 29/24 F1 - 3/40 F2 + 1/168 F3, term by term (inh_v_qz_int_1.F90) */
+#define DZ1L(lead, F, I)                                                                       \
+  (((((lead) - (real)31 / (real)24 * F(I, 0)) + (real)29 / (real)24 * F(I, 1)) -               \
+    (real)3 / (real)40 * F(I, 2)) + (real)1 / (real)168 * F(I, 3))
+
+/* INH_QX_INT_0, inh_qx_int_0.F90:43-71 (4th-order branch) */
+static void inh_qx_int_0(orc *o) {
+  const real A = o->A, B = o->B;
+  for (int I = 4; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 0);
+    QXF(I, 0) = rexp(-TAB(T_REL3_QX, jm)) * QXF(I, 0) -
+                TAB(T_REL1_QX, jm) *
+                    TXZ_Z0(A * (TXXF(I + 1, 0) - TXXF(I - 2, 0)) + B * (TXXF(I, 0) - TXXF(I - 1, 0)), I) -
+                TAB(T_REL2_QX, jm) * (A * (PRF(I + 1, 0) - PRF(I - 2, 0)) + B * (PRF(I, 0) - PRF(I - 1, 0)));
+  }
+}
+
+/* INH_V_UV_INT_0, inh_v_uv_int_0.F90:43-72 (4th-order branch) */
+static void inh_v_uv_int_0(orc *o) {
+  const real A = o->A, B = o->B;
+  for (int I = 4; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 0);
+    UM(I, 0) = UM(I, 0) +
+               (TAB(T_REL2_U, jm) / TAB(T_REL3_QX, jm)) * ((real)1 - rexp(-TAB(T_REL3_QX, jm))) *
+                   QXPF(I, 0) +
+               TAB(T_REL1_U, jm) *
+                   TXZ_Z0(A * (TXXF(I + 1, 0) - TXXF(I - 2, 0)) + B * (TXXF(I, 0) - TXXF(I - 1, 0)), I) +
+               TAB(T_REL3_U, jm) * (A * (PRF(I + 1, 0) - PRF(I - 2, 0)) + B * (PRF(I, 0) - PRF(I - 1, 0)));
+  }
+}
+
+/* INH_QZ_INT_0, inh_qz_int_0.F90:43-62 */
+static void inh_qz_int_0(orc *o) {
+  for (int I = 3; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 0);
+    QZF(I, 0) = rexp(-TAB(T_REL3_QZ, jm)) * QZF(I, 0) - TAB(T_REL1_QZ, jm) * DZ0(TZZF, I) -
+                TAB(T_REL2_QZ, jm) * DZ0(PRF, I);
+  }
+}
+
+/* INH_V_W_INT_0, inh_v_w_int_0.F90:43-63 */
+static void inh_v_w_int_0(orc *o) {
+  for (int I = 3; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 0);
+    WM(I, 0) = WM(I, 0) +
+               (TAB(T_REL2_W, jm) / TAB(T_REL3_QZ, jm)) * ((real)1 - rexp(-TAB(T_REL3_QZ, jm))) *
+                   QZPF(I, 0) +
+               TAB(T_REL1_W, jm) * DZ0(TZZF, I) + TAB(T_REL3_W, jm) * DZ0(PRF, I);
+  }
+}
+
+/* INH_QX_2ND_0, inh_qx_2nd_0.F90:55-104, and INH_V_U_2ND_0, inh_v_u_2nd_0.F90:57-106:
+ * the side strips of row 0 (the z term is TXZ(i,1), TXZ(i,0) being 0) */
+static void inh_qxu_2nd_0(orc *o) {
+  const int L = 0;
+  for (int side = 0; side < 2; ++side) {
+    int lo = side ? RIGHT_LO : LEFT_UQX_LO, hi = side ? RIGHT_HI : LEFT_UQX_HI;
+    for (int I = lo; I <= hi; ++I) {
+      int jm = JMF(I, L);
+      int j = side ? I - o->MX1 : 3 - I;
+      real oda = side ? XL2(o->ODA2_XH, j, L) : XL2(o->ODA2_XL, j, L);
+      real odb = side ? XL2(o->ODB2_XH, j, L) : XL2(o->ODB2_XL, j, L);
+      real odc = side ? XL2(o->ODC2_XH, j, L) : XL2(o->ODC2_XL, j, L);
+      real odd = side ? XL2(o->ODD2_XH, j, L) : XL2(o->ODD2_XL, j, L);
+      real *G = side ? &MR1(o->GXXH, W_R1, I, L) : &ML(o->GXXL, W_L3, I, L);
+      real *Hh = side ? &MR1(o->HXXH, W_R1, I, L) : &ML(o->HXXL, W_L3, I, L);
+      real DXXXA = TXXF(I, L) - TXXF(I - 1, L);
+      real PXXXA = PRF(I, L) - PRF(I - 1, L);
+      QXF(I, L) = rexp(-TAB(T_REL3_QX, jm)) * QXF(I, L) -
+                  TAB(T_REL1_QX, jm) * (oda * (odd * DXXXA + *G) + TXZF(I, 1)) -
+                  TAB(T_REL2_QX, jm) * (oda * (odd * PXXXA + *Hh));
+      *G = odb * *G + odc * DXXXA;
+      *Hh = odb * *Hh + odc * PXXXA;
+    }
+  }
+  for (int side = 0; side < 2; ++side) {
+    int lo = side ? RIGHT_LO : LEFT_UQX_LO, hi = side ? RIGHT_HI : LEFT_UQX_HI;
+    for (int I = lo; I <= hi; ++I) {
+      int jm = JMF(I, L);
+      int j = side ? I - o->MX1 : 3 - I;
+      real oda = side ? XL2(o->ODA2_XH, j, L) : XL2(o->ODA2_XL, j, L);
+      real odb = side ? XL2(o->ODB2_XH, j, L) : XL2(o->ODB2_XL, j, L);
+      real odc = side ? XL2(o->ODC2_XH, j, L) : XL2(o->ODC2_XL, j, L);
+      real odd = side ? XL2(o->ODD2_XH, j, L) : XL2(o->ODD2_XL, j, L);
+      real *R = side ? &MR1(o->RXXH, W_R1, I, L) : &ML(o->RXXL, W_L3, I, L);
+      real *S = side ? &MR1(o->SXXH, W_R1, I, L) : &ML(o->SXXL, W_L3, I, L);
+      real DXXXA = TXXF(I, L) - TXXF(I - 1, L);
+      real PXXXA = PRF(I, L) - PRF(I - 1, L);
+      UM(I, L) = UM(I, L) +
+                 (TAB(T_REL2_U, jm) / TAB(T_REL3_QX, jm)) * ((real)1 - rexp(-TAB(T_REL3_QX, jm))) *
+                     QXPF(I, L) +
+                 TAB(T_REL1_U, jm) * (oda * (odd * DXXXA + *R) + TXZF(I, 1)) +
+                 TAB(T_REL3_U, jm) * (oda * (odd * PXXXA + *S));
+      *R = odb * *R + odc * DXXXA;
+      *S = odb * *S + odc * PXXXA;
+    }
+  }
+}
+
+/* INH_QZ_2ND_0, inh_qz_2nd_0.F90:48-80, and INH_V_W_2ND_0, inh_v_w_2nd_0.F90:50-84:
+ * the side strips of row 0 (no x-derivative term, so no PML memory) */
+static void inh_qzw_2nd_0(orc *o) {
+  const int MXT1 = o->MX - 1;
+  for (int side = 0; side < 2; ++side) {
+    int lo = side ? MXT1 : 2 - o->T, hi = side ? MXT1 + o->T : 2;
+    for (int I = lo; I <= hi; ++I) {
+      int jm = JMF(I, 0);
+      QZF(I, 0) = rexp(-TAB(T_REL3_QZ, jm)) * QZF(I, 0) -
+                  TAB(T_REL1_QZ, jm) * ((real)3 * TZZF(I, 0) - (real)1 / (real)3 * TZZF(I, 1)) -
+                  TAB(T_REL2_QZ, jm) * ((real)3 * PRF(I, 0) - (real)1 / (real)3 * PRF(I, 1));
+    }
+  }
+  for (int side = 0; side < 2; ++side) {
+    int lo = side ? MXT1 : 2 - o->T, hi = side ? MXT1 + o->T : 2;
+    for (int I = lo; I <= hi; ++I) {
+      int jm = JMF(I, 0);
+      WM(I, 0) = WM(I, 0) +
+                 (TAB(T_REL2_W, jm) / TAB(T_REL3_QZ, jm)) * ((real)1 - rexp(-TAB(T_REL3_QZ, jm))) *
+                     QZPF(I, 0) +
+                 TAB(T_REL1_W, jm) * ((real)3 * TZZF(I, 0) - (real)1 / (real)3 * TZZF(I, 1)) +
+                 TAB(T_REL3_W, jm) * ((real)3 * PRF(I, 0) - (real)1 / (real)3 * PRF(I, 1));
+    }
+  }
+}
+
+/* HPLANE_0_VELOCITY without stress imaging, hplane_0_velocity.F90:62-179, then
+ * STO_QXP/STO_QZP of row 0 (:181-195) */
+static void hplane_0_velocity(orc *o) {
+  inh_qx_int_0(o);
+  inh_v_uv_int_0(o);
+  inh_qz_int_0(o);
+  inh_v_w_int_0(o);
+  inh_qxu_2nd_0(o);
+  inh_qzw_2nd_0(o);
+  sto_qp(o, 0);
+}
+
+/* INH_V_QZ_INT_1, inh_v_qz_int_1.F90:49-79, and INH_V_W_INT_1, inh_v_w_int_1.F90:51-82
+ * (4th-order branch: the reference guards its 2nd-order branch with SEDOND) */
+static void inh_qzw_int_1(orc *o) {
+  const real A = o->A, B = o->B;
+  for (int I = 3; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 1);
+    QZF(I, 1) = rexp(-TAB(T_REL3_QZ, jm)) * QZF(I, 1) -
+                TAB(T_REL1_QZ, jm) *
+                    DZ1L(A * (TXZF(I + 2, 1) - TXZF(I - 1, 1)) + B * (TXZF(I + 1, 1) - TXZF(I, 1)), TZZF, I) -
+                TAB(T_REL2_QZ, jm) * DZ1(PRF, I);
+  }
+  for (int I = 3; I <= o->MX - 2; ++I) {
+    int jm = JMF(I, 1);
+    WM(I, 1) = WM(I, 1) +
+               (TAB(T_REL2_W, jm) / TAB(T_REL3_QZ, jm)) * ((real)1 - rexp(-TAB(T_REL3_QZ, jm))) *
+                   QZPF(I, 1) +
+               TAB(T_REL1_W, jm) *
+                   DZ1L(A * (TXZF(I + 2, 1) - TXZF(I - 1, 1)) + B * (TXZF(I + 1, 1) - TXZF(I, 1)), TZZF, I) +
+               TAB(T_REL3_W, jm) * DZ1(PRF, I);
+  }
+}
+
+/* HPLANE_1_VELOCITY without stress imaging, hplane_1_velocity.F90:79-207, then
+ * STO_QXP/STO_QZP of row 1 */
+static void hplane_1_velocity(orc *o) {
+  inh_qx_int(o, 1);
+  inh_v_uv_int(o, 1);
+  inh_qzw_int_1(o);
+  inh_qx_2nd(o, 1);
+  inh_v_u_2nd(o, 1);
+  inh_qz_2nd(o, 1);
+  inh_v_w_2nd(o, 1);
+  sto_qp(o, 1);
+}
+
 /* Bottom-boundary store, time_loop.F90:133-155 */
 static void store_bottom(orc *o) {
   const int NX = o->NX;
@@ -749,7 +1016,7 @@ static void sample_receivers(orc *o, int itile) {
   }
 }
 
-/* One DO ITILE iteration, time_loop.F90:91-190 (serial, stress imaging). */
+/* One DO ITILE iteration, time_loop.F90:91-190 (serial build). */
 static void one_step(orc *o, int itile) {
   const int MZ = o->MZ, PMLZH1 = o->PMLZH - 1;
   rowtmp t = {o->EXX, o->EZZ, o->EXZ, o->QXX, o->QZZ};
@@ -759,7 +1026,12 @@ static void one_step(orc *o, int itile) {
   for (int L = 2; L <= o->LIN_Z - 1; ++L) hplane_ic_stress(o, L, &t, 0, 0, 0);
   hplane_ic_stress(o, o->LIN_Z, &t, 1, S, F);
   for (int L = o->LIN_Z + 1; L <= MZ - 2; ++L) hplane_ic_stress(o, L, &t, 0, 0, 0);
-  hplane_01_stress_si(o);
+  if (o->SI) {
+    hplane_01_stress_si(o);
+  } else {
+    hplane_0_stress(o, &t);
+    hplane_1_stress(o, &t);
+  }
   hplane_mz1_stress(o, &t);
   for (int L = MZ; L <= PMLZH1; ++L) hplane_pml_stress(o, L, L - MZ + 1, &t);
   hplane_pmlzh0_stress(o, &t);
@@ -767,7 +1039,12 @@ static void one_step(orc *o, int itile) {
   store_bottom(o);
   /* velocity half-step */
   for (int L = 2; L <= MZ - 2; ++L) hplane_ic_velocity(o, L);
-  hplane_1_velocity_si(o);
+  if (o->SI) {
+    hplane_1_velocity_si(o);
+  } else {
+    hplane_0_velocity(o);
+    hplane_1_velocity(o);
+  }
   for (int L = MZ - 1; L <= PMLZH1; ++L) hplane_pml_velocity(o, L, L - MZ + 1);
   hplane_pmlzh0_velocity(o);
   sample_receivers(o, itile);
@@ -896,12 +1173,13 @@ EXPORT int oracle_create(const oracle_desc *d, orc **out) {
   if (!d || !out) return -1;
   *out = NULL;
   if (d->mx < 8 || d->mz < 6 || d->tpml < 1 || d->tpml > 256 || d->jmnum < 1) return -2;
-  if (!d->stress_imaging) return -3; /* the free-surface variant implemented here is SI */
+  if (d->variant & ~(V_AFDA_ZZ | V_AFDA_XZ)) return -3;
   if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
       d->lin_x > d->mx - 1 + d->tpml)
     return -4;
   orc *o = (orc *)zalloc(sizeof(orc));
-  o->MX = d->mx; o->MZ = d->mz; o->T = d->tpml; o->SI = d->stress_imaging;
+  o->MX = d->mx; o->MZ = d->mz; o->T = d->tpml; o->SI = d->stress_imaging != 0;
+  o->AFDA_ZZ = (d->variant & V_AFDA_ZZ) != 0; o->AFDA_XZ = (d->variant & V_AFDA_XZ) != 0;
   o->LIN_X = d->lin_x; o->LIN_Z = d->lin_z; o->KTBO = d->ktbo; o->MT2 = d->mt2;
   o->JMNUM = d->jmnum; o->next_step = 1;
   o->H = (real)d->h; o->DT = (real)d->dt;

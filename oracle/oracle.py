@@ -26,7 +26,7 @@ class Desc(C.Structure):
     _fields_ = [("mx", C.c_int32), ("mz", C.c_int32), ("tpml", C.c_int32),
                 ("stress_imaging", C.c_int32), ("lin_x", C.c_int32), ("lin_z", C.c_int32),
                 ("ktbo", C.c_int32), ("mt2", C.c_int32), ("jmnum", C.c_int32),
-                ("reserved", C.c_int32), ("h", C.c_double), ("dt", C.c_double)]
+                ("variant", C.c_int32), ("h", C.c_double), ("dt", C.c_double)]
 
 
 _LIBS = {}
@@ -97,7 +97,7 @@ class OracleSim:
         self.job = job
         self.nz, self.nx = hostsetup.grid_shape(job)
         d = Desc(job.mx, job.mzp, job.tpml, int(job.stress_imaging), job.lin_x, job.lin_z,
-                 job.ktbo, job.mt2, job.table.shape[0], 0, job.h,
+                 job.ktbo, job.mt2, job.table.shape[0], int(job.afda), job.h,
                  float(np.float32(job.dt)) if precision == "f32" else job.dt)
         h = C.c_void_p()
         rc = self.lib.oracle_create(C.byref(d), C.byref(h))

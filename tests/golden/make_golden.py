@@ -14,6 +14,7 @@ reference's own input files, runs both binaries, and stores
 * the receiver files REC_UW_j, REC_Q_j, REC_STRESS_j (6 significant digits).
 
 Run in the container that holds the reference: python tests/golden/make_golden.py
+[case ...] (all cases without arguments).
 """
 from __future__ import annotations
 
@@ -37,11 +38,18 @@ REF = os.path.join(ROOT, "oracle", "_ref", "fd2d")
 REF_FMA = os.path.join(ROOT, "oracle", "_ref", "fd2d_fma")
 
 SCALARS = ("name", "mx", "mz", "h", "dt", "mt", "x_source", "z_source", "tpml", "stress_imaging",
-           "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo")
+           "afda", "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo")
 
 
 def cases():
+    # Without stress imaging the reference stops on its first step: the
+    # interface block of INH_QZ_2ND_0 (mod_interfaces.F90:702-709) lists PL, PXH
+    # and MX1, which the subroutine (inh_qz_2nd_0.F90:1-8) does not take, so
+    # the keyword call at hplane_0_velocity.F90:163-171 hands it shifted
+    # arguments; the run faults inside inh_qz_2nd_0.F90:43. Those cases have no
+    # reference fixture.
     out = {name: fn() for name, fn in jobs.ALL.items()}
+    out = {name: job for name, job in out.items() if job.stress_imaging}
     # KTBO != 0 switches the bottom PML damping off (ppml.F90:261), which makes
     # ODC = 0/0 unless WC > 0; the case therefore also exercises WC and SF.
     k = jobs.tiny_layered(steps=150, seed=11)
@@ -80,14 +88,17 @@ def run(exe, job, d, snap_steps):
 
 
 def main():
-    if not (os.path.exists(REF) and os.path.exists(REF_FMA)):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    only = set(sys.argv[1:])
     for name, job in cases().items():
+        if only and name not in only:
+            continue
+        exe, exe_fma = REF, REF_FMA
         steps = job.mt2
         snap_steps = sorted({1, 2, 10, steps // 2, steps})
         with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
-            snaps, recs = run(REF, job, d1, snap_steps)
-            snaps_fma, _ = run(REF_FMA, job, d2, [steps])
+            snaps, recs = run(exe, job, d1, snap_steps)
+            snaps_fma, _ = run(exe_fma, job, d2, [steps])
         meta = {k: getattr(job, k) for k in SCALARS}
         meta["receivers"] = [list(map(int, r)) for r in job.receivers]
         path = os.path.join(HERE, f"{name}.npz")

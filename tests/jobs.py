@@ -36,7 +36,31 @@ def tiny_edges(steps=150, tpml=5, mx=36, mz=24, f0=90.0, seed=7) -> Job:
     return models.make_job("EDGES", rocks, jm, h, steps, f0, xs, zs, recs, tpml=tpml, key_tlp=True)
 
 
-ALL = {"layered": tiny_layered, "homog": tiny_homog, "edges": tiny_edges}
+def tiny_layered_nosi(steps=120, afda=0, **kw) -> Job:
+    """Explicit free surface (no stress imaging, MZ' = MZ): the surface rows 0
+    and 1 use their own one-sided stencils. Source 3 cells deep (rows 0 and 1
+    cannot hold it), receivers on rows 0 and 1. afda selects the reference's
+    AFDA_ZZ / AFDA_XZ stencil options."""
+    job = tiny_layered(steps=steps, **kw)
+    job.stress_imaging = False
+    job.afda = afda
+    job.z_source = 3.0 * job.h
+    job.receivers = [(r[0], k % 2, r[2], k % 2) for k, r in enumerate(job.receivers)]
+    return job
+
+
+def tiny_edges_nosi(steps=150, **kw) -> Job:
+    """The edge case without stress imaging: source in the left PML strip on
+    row MZ'-2 = MZ-2."""
+    job = tiny_edges(steps=steps, **kw)
+    job.stress_imaging = False
+    job.z_source = (job.mz - 2) * job.h
+    return job
+
+
+ALL = {"layered": tiny_layered, "homog": tiny_homog, "edges": tiny_edges,
+       "layered_nosi": tiny_layered_nosi, "edges_nosi": tiny_edges_nosi,
+       "layered_nosi_afda": lambda: tiny_layered_nosi(afda=3)}
 
 
 def medium_layered(steps=200, tpml=10, mx=200, mz=120, seed=5, f0=40.0) -> Job:
@@ -57,15 +81,20 @@ def medium_deep(steps=200, tpml=12, mx=230, mz=140, f0=45.0) -> Job:
 GPU_EXTRA = {"medium": medium_layered, "deep": medium_deep}
 
 
-def wide_deep(steps=160, tpml=9, mx=600, mz=170, f0=30.0) -> Job:
-    """Wide enough for the fused interior blocks (256 columns); correlated random
-    model. The source cell (279, 68) lies in the last column and row owned by a
-    fused block, so the neighbouring blocks recompute it in their halos."""
+def wide_deep(steps=160, tpml=9, mx=1100, mz=170, f0=30.0, stress_imaging=True) -> Job:
+    """Wide enough for two fused tiles of 512 columns side by side and two rows
+    of 128-row blocks; correlated random model. The source cell (535, 132) is
+    the last column and row of the first tile, so the tiles right of and below
+    it recompute it in their halos. With TPML = 9 the tiles start at column 24."""
     job = models.layered_job(name="WIDE", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=4, seed=13,
                              tpml=tpml, f0=f0, n_rec=50, field_std=0.08, corr_len=16.0, levels=5)
-    job.x_source = 279.0 * 2.0
-    job.z_source = 66.0 * 2.0
+    job.stress_imaging = stress_imaging
+    job.x_source = 535.0 * 2.0
+    job.z_source = (132.0 - (2.0 if stress_imaging else 0.0)) * 2.0
+    if not stress_imaging:
+        job.receivers = [(r[0], 0, r[2], 0) for r in job.receivers]
     return job
 
 
 GPU_EXTRA["wide"] = wide_deep
+GPU_EXTRA["wide_nosi"] = lambda: wide_deep(stress_imaging=False)
