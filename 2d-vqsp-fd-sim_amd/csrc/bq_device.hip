@@ -43,6 +43,8 @@ constexpr int kEdgeBlock = 256;
 
 struct KParams {
   int MX, MZ, T, PMLL, PMLZH, NX, P, OFF, LIN_X, LIN_Z, MR;
+  int SI;                // stress imaging (else explicit surface rows 0 and 1)
+  int AFDA_ZZ, AFDA_XZ;  // one-sided surface options of the explicit surface
   float DT, A, B, AH, BH, CH, BC;
   // fields of the current step (read) and of the next step (written); the two
   // sets trade places after every step
@@ -70,6 +72,16 @@ struct KParams {
 #define ZH1(t, j, ix) (k.z1[t][(ix) * (T + 1) + (j)])
 #define ZH2(t, j, ix) (k.z2[t][(ix) * (T + 2) + (j)])
 
+// One-sided surface stencils of the explicit free surface (binary32 quotients,
+// as the reference's constant expressions fold them).
+constexpr float c17_24 = 17.0f / 24.0f, c3_8 = 3.0f / 8.0f, c5_24 = 5.0f / 24.0f;
+constexpr float c1_24 = 1.0f / 24.0f, c11_12 = 11.0f / 12.0f;
+constexpr float c35_8 = 35.0f / 8.0f, c35_24 = 35.0f / 24.0f, c21_40 = 21.0f / 40.0f;
+constexpr float c5_56 = 5.0f / 56.0f, c31_24 = 31.0f / 24.0f, c29_24 = 29.0f / 24.0f;
+constexpr float c3_40 = 3.0f / 40.0f, c1_168 = 1.0f / 168.0f, c1_3 = 1.0f / 3.0f;
+constexpr float c1_22 = 1.0f / 22.0f, c577_528 = 577.0f / 528.0f, c201_176 = 201.0f / 176.0f;
+constexpr float c9_176 = 9.0f / 176.0f, c1_528 = 1.0f / 528.0f;
+
 // ------------------------------------------------------------------ stress
 // One cell (i,l) of the stress half-step, l in [2, PMLZH]:
 //   l in [2, MZ'-2]  HPLANE_IC_STRESS / HPLANE_LIN_STRESS (hplane_ic_stress.F90:48-117)
@@ -77,6 +89,10 @@ struct KParams {
 //   l in [MZ', PMLZH-1]  HPLANE_PML_STRESS, LL = l-MZ'+1 (hplane_pml_stress.F90:53-134)
 //   l = PMLZH        HPLANE_PMLZH0_STRESS, TXZ only (hplane_pmlzh0_stress.F90:43-68)
 // and the stress-imaging rows (hplane_0_stress.F90:47-51, hplane_1_stress.F90:44-49).
+// Without stress imaging also l = 0 and 1 (HPLANE_0_STRESS, HPLANE_1_STRESS,
+// hplane_0_stress.F90:57-119, hplane_1_stress.F90:51-126): EXZ = 0 on row 0,
+// STRAIN_UW_XZ_1 on row 1 (strain_uw_xz_1.F90:62-114), STRAIN_UW_ZZ_0 /
+// STRAIN_Q_ZZ_0 on row 0 (strain_uw_zz_0.F90:45-62, strain_q_zz_0.F90:40-57).
 __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const int l,
                                             const int itile) {
   const int T = k.T, MX = k.MX, MZ = k.MZ, PMLZH = k.PMLZH, NX = k.NX;
@@ -94,7 +110,9 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
   float txz = FA(k.TXZ0, i, l);
   if (i >= 2 - T && i <= MX + T) {
     float exz;
-    if (!zpml) {
+    if (l == 0) {
+      exz = 0.0f;
+    } else if (!zpml) {
       if (i <= 2) {
         const float dzxa = C * (FA(k.W0, i, l) - FA(k.W0, i - 1, l));
         float &pm = k.PZXL[(size_t)l * (T + 2) + ix];
@@ -102,8 +120,20 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
         exz = C * (FA(k.U0, i, l) - FA(k.U0, i, l - 1)) + XL2(0, j, l) * (XL2(3, j, l) * dzxa + pm);
         pm = XL2(1, j, l) * pm + XL2(2, j, l) * dzxa;
       } else if (i <= MX - 1) {
-        exz = k.AH * (FA(k.U0, i, l + 1) - FA(k.U0, i, l - 2) + FA(k.W0, i + 1, l) - FA(k.W0, i - 2, l)) +
-              k.BH * (FA(k.U0, i, l) - FA(k.U0, i, l - 1) + FA(k.W0, i, l) - FA(k.W0, i - 1, l));
+        if (l != 1) {
+          exz = k.AH * (FA(k.U0, i, l + 1) - FA(k.U0, i, l - 2) + FA(k.W0, i + 1, l) - FA(k.W0, i - 2, l)) +
+                k.BH * (FA(k.U0, i, l) - FA(k.U0, i, l - 1) + FA(k.W0, i, l) - FA(k.W0, i - 1, l));
+        } else if (!k.AFDA_XZ) {
+          exz = k.AH * (FA(k.W0, i + 1, 1) - FA(k.W0, i - 2, 1)) + k.BH * (FA(k.W0, i, 1) - FA(k.W0, i - 1, 1)) +
+                C * (FA(k.U0, i, 1) - FA(k.U0, i, 0));
+        } else {
+          const float duikm1 = -(k.AH * (FA(k.W0, i + 1, 0) - FA(k.W0, i - 2, 0))) -
+                               k.BH * (FA(k.W0, i, 0) - FA(k.W0, i - 1, 0));
+          exz = k.AH * (FA(k.W0, i + 1, 1) - FA(k.W0, i - 2, 1)) + k.BH * (FA(k.W0, i, 1) - FA(k.W0, i - 1, 1)) -
+                c1_22 * duikm1 +
+                C * (-(c577_528 * FA(k.U0, i, 0)) + c201_176 * FA(k.U0, i, 1) - c9_176 * FA(k.U0, i, 2) +
+                     c1_528 * FA(k.U0, i, 3));
+        }
       } else {
         const float dzxa = C * (FA(k.W0, i, l) - FA(k.W0, i - 1, l));
         float &pm = k.PZXH[(size_t)l * (T + 1) + (i - MX)];
@@ -134,9 +164,9 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
     txz = txz + DT * (s0.x * exz);
   }
   // TXZ(:,2) = 0 after the images are taken (hplane_1_stress.F90:48)
-  FA(k.TXZ1, i, l) = (l == 2) ? 0.0f : txz;
-  if (l == 3) FA(k.TXZ1, i, 1) = -txz;
-  if (l == 4) FA(k.TXZ1, i, 0) = -txz;
+  FA(k.TXZ1, i, l) = (k.SI && l == 2) ? 0.0f : txz;
+  if (k.SI && l == 3) FA(k.TXZ1, i, 1) = -txz;
+  if (k.SI && l == 4) FA(k.TXZ1, i, 0) = -txz;
   if (zh0) return;
 
   // ---- normal strains and TXX, TZZ, PRES: STRAIN_UW_XX / STRAIN_Q_XX
@@ -175,7 +205,16 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
       k.PXXH[mo] = b * pu + c * du;
       k.OXXH[mo] = b * pq + c * dq;
     }
-    if (!pmlrow && l != MZ - 1) {
+    if (l == 0) {
+      ezz = (C * (FA(k.W0, i, 1) - FA(k.W0, i, 0)));
+      qzz = (C * (FA(k.QZ0, i, 1) - FA(k.QZ0, i, 0)));
+      if (k.AFDA_ZZ && i >= 2 && i <= MX - 1) {
+        ezz = C * (-(c11_12 * FA(k.W0, i, 0)) + c17_24 * FA(k.W0, i, 1) + c3_8 * FA(k.W0, i, 2) -
+                   c5_24 * FA(k.W0, i, 3) + c1_24 * FA(k.W0, i, 4));
+        qzz = C * (-(c11_12 * FA(k.QZ0, i, 0)) + c17_24 * FA(k.QZ0, i, 1) + c3_8 * FA(k.QZ0, i, 2) -
+                   c5_24 * FA(k.QZ0, i, 3) + c1_24 * FA(k.QZ0, i, 4));
+      }
+    } else if (!pmlrow && l != MZ - 1) {
       ezz = (C * (FA(k.W0, i, l + 1) - FA(k.W0, i, l)));
       qzz = (C * (FA(k.QZ0, i, l + 1) - FA(k.QZ0, i, l)));
       if (i >= 2 && i <= MX - 1) {
@@ -213,8 +252,8 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
     FA(k.TZZ1, i, l) = tzz;
     FA(k.PR1, i, l) = pr;
   }
-  if (l == 2) { FA(k.TZZ1, i, 1) = -tzz; FA(k.PR1, i, 1) = -pr; }
-  if (l == 3) { FA(k.TZZ1, i, 0) = -tzz; FA(k.PR1, i, 0) = -pr; }
+  if (k.SI && l == 2) { FA(k.TZZ1, i, 1) = -tzz; FA(k.PR1, i, 1) = -pr; }
+  if (k.SI && l == 3) { FA(k.TZZ1, i, 0) = -tzz; FA(k.PR1, i, 0) = -pr; }
 }
 
 // ---------------------------------------------------------------- velocity
@@ -225,6 +264,10 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
 //                          INH_V_U_2ND (inh_qx_2nd.F90:54-103, inh_v_u_2nd.F90:56-104)
 //                          in the side PML; likewise QZ/W
 //   l in [MZ'-1, PMLZH-1]  HPLANE_PML_VELOCITY, LL = l-MZ'+1 (hplane_pml_velocity.F90:64-180)
+// and without stress imaging l = 0 (HPLANE_0_VELOCITY, hplane_0_velocity.F90:62-179:
+// INH_QX_INT_0, INH_V_UV_INT_0, INH_QZ_INT_0, INH_V_W_INT_0, INH_*_2ND_0) and
+// l = 1 (HPLANE_1_VELOCITY, hplane_1_velocity.F90:79-207: the row-2 forms except
+// INH_V_QZ_INT_1 / INH_V_W_INT_1, inh_v_qz_int_1.F90:49-79, inh_v_w_int_1.F90:51-82).
 __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, const int l) {
   const int T = k.T, MX = k.MX, NX = k.NX;
   const int ix = i - k.PMLL;
@@ -243,8 +286,15 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
     const bool inner = (i >= 4 && i <= MX - 2);
     if (!pmlrow) {
       if (inner) {
-        const float sx = A * (FA(k.TXX1, i + 1, l) - FA(k.TXX1, i - 2, l) + FA(k.TXZ1, i, l + 2) - FA(k.TXZ1, i, l - 1)) +
-                         B * (FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l) + FA(k.TXZ1, i, l + 1) - FA(k.TXZ1, i, l));
+        float sx;
+        if (l != 0) {
+          sx = A * (FA(k.TXX1, i + 1, l) - FA(k.TXX1, i - 2, l) + FA(k.TXZ1, i, l + 2) - FA(k.TXZ1, i, l - 1)) +
+               B * (FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l) + FA(k.TXZ1, i, l + 1) - FA(k.TXZ1, i, l));
+        } else {  // TXZ(i,0) = 0 on the surface: one-sided z-derivative from rows 1-4
+          sx = A * (FA(k.TXX1, i + 1, 0) - FA(k.TXX1, i - 2, 0)) + B * (FA(k.TXX1, i, 0) - FA(k.TXX1, i - 1, 0)) +
+               c17_24 * FA(k.TXZ1, i, 1) + c3_8 * FA(k.TXZ1, i, 2) - c5_24 * FA(k.TXZ1, i, 3) +
+               c1_24 * FA(k.TXZ1, i, 4);
+        }
         const float px = A * (FA(k.PR1, i + 1, l) - FA(k.PR1, i - 2, l)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i - 1, l));
         qn = v0.x * q - v0.y * sx - v0.z * px;
         un = u + v0.w * q + v1.x * sx + v1.y * px;
@@ -260,8 +310,13 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         const float dxx = FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l);
         const float pxx = FA(k.PR1, i, l) - FA(k.PR1, i - 1, l);
         const float tz1 = FA(k.TXZ1, i, l + 1), tz0 = FA(k.TXZ1, i, l);
-        qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1 - tz0) - v0.z * (oa * (od * pxx + hh));
-        un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1 - tz0) + v1.y * (oa * (od * pxx + s));
+        if (l != 0) {
+          qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1 - tz0) - v0.z * (oa * (od * pxx + hh));
+          un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1 - tz0) + v1.y * (oa * (od * pxx + s));
+        } else {  // row 0: the z term is TXZ(i,1) alone (inh_qx_2nd_0.F90, inh_v_u_2nd_0.F90)
+          qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1) - v0.z * (oa * (od * pxx + hh));
+          un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1) + v1.y * (oa * (od * pxx + s));
+        }
         G[mo] = ob * g + oc * dxx;
         H[mo] = ob * hh + oc * pxx;
         R[mo] = ob * r + oc * dxx;
@@ -311,9 +366,30 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
     if (!pmlrow) {
       if (inner) {
         // INH_QZ_INT (inh_qz_int.F90:41-55), INH_V_W_INT (inh_v_w_int.F90:43-58)
-        const float sz = A * (FA(k.TZZ1, i, l + 1) - FA(k.TZZ1, i, l - 2) + FA(k.TXZ1, i + 2, l) - FA(k.TXZ1, i - 1, l)) +
-                         B * (FA(k.TZZ1, i, l) - FA(k.TZZ1, i, l - 1) + FA(k.TXZ1, i + 1, l) - FA(k.TXZ1, i, l));
-        const float pz = A * (FA(k.PR1, i, l + 1) - FA(k.PR1, i, l - 2)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i, l - 1));
+        float sz, pz;
+        if (l >= 2) {
+          sz = A * (FA(k.TZZ1, i, l + 1) - FA(k.TZZ1, i, l - 2) + FA(k.TXZ1, i + 2, l) - FA(k.TXZ1, i - 1, l)) +
+               B * (FA(k.TZZ1, i, l) - FA(k.TZZ1, i, l - 1) + FA(k.TXZ1, i + 1, l) - FA(k.TXZ1, i, l));
+          pz = A * (FA(k.PR1, i, l + 1) - FA(k.PR1, i, l - 2)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i, l - 1));
+        } else if (l == 0) {  // INH_QZ_INT_0, INH_V_W_INT_0: TZZ, PRES vanish at the surface
+          sz = c35_8 * FA(k.TZZ1, i, 0) - c35_24 * FA(k.TZZ1, i, 1) + c21_40 * FA(k.TZZ1, i, 2) -
+               c5_56 * FA(k.TZZ1, i, 3);
+          pz = c35_8 * FA(k.PR1, i, 0) - c35_24 * FA(k.PR1, i, 1) + c21_40 * FA(k.PR1, i, 2) -
+               c5_56 * FA(k.PR1, i, 3);
+        } else {  // INH_V_QZ_INT_1, INH_V_W_INT_1
+          sz = A * (FA(k.TXZ1, i + 2, 1) - FA(k.TXZ1, i - 1, 1)) + B * (FA(k.TXZ1, i + 1, 1) - FA(k.TXZ1, i, 1)) -
+               c31_24 * FA(k.TZZ1, i, 0) + c29_24 * FA(k.TZZ1, i, 1) - c3_40 * FA(k.TZZ1, i, 2) +
+               c1_168 * FA(k.TZZ1, i, 3);
+          pz = -(c31_24 * FA(k.PR1, i, 0)) + c29_24 * FA(k.PR1, i, 1) - c3_40 * FA(k.PR1, i, 2) +
+               c1_168 * FA(k.PR1, i, 3);
+        }
+        qn = v1.z * q - v1.w * sz - v2.x * pz;
+        wn = w + v2.y * q + v2.z * sz + v2.w * pz;
+      } else if (l == 0) {
+        // INH_QZ_2ND_0, INH_V_W_2ND_0 (inh_qz_2nd_0.F90:48-80, inh_v_w_2nd_0.F90:50-84):
+        // no x term on the surface row, so no PML memory
+        const float sz = 3.0f * FA(k.TZZ1, i, 0) - c1_3 * FA(k.TZZ1, i, 1);
+        const float pz = 3.0f * FA(k.PR1, i, 0) - c1_3 * FA(k.PR1, i, 1);
         qn = v1.z * q - v1.w * sz - v2.x * pz;
         wn = w + v2.y * q + v2.z * sz + v2.w * pz;
       } else {
@@ -850,7 +926,9 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
   const int ix0 = blockIdx.x * kEdgeBlock;
   const int ix = ix0 + threadIdx.x;
   // phase 1: U(i,1), QX(i,1) for the block's columns and one halo column
-  for (int t = threadIdx.x; t <= kEdgeBlock; t += kEdgeBlock) {
+  // (stress imaging only; without it row 1 is a per-cell row of k_velocity)
+  const bool si = k.SI != 0;
+  for (int t = threadIdx.x; si && t <= kEdgeBlock; t += kEdgeBlock) {
     const int jx = ix0 + t;
     const int i = jx + k.PMLL;
     if (jx < NX && i >= 3 - T && i <= MX - 1 + T) {
@@ -860,15 +938,15 @@ __global__ __launch_bounds__(kEdgeBlock) void k_edges(KParams k) {
       sq[t] = q1;
     }
   }
-  __syncthreads();
+  if (si) __syncthreads();
   if (ix < NX) {
     const int i = ix + k.PMLL;
-    if (i >= 3 - T && i <= MX - 1 + T) {
+    if (si && i >= 3 - T && i <= MX - 1 + T) {
       FA(k.U1, i, 1) = su[threadIdx.x];
       FA(k.QX1, i, 1) = sq[threadIdx.x];
     }
     // phase 2
-    if (i >= 3 - T && i <= MX - 2 + T) {
+    if (si && i >= 3 - T && i <= MX - 2 + T) {
       const int jm = k.JM[ix] - 1;  // JM(I,0)
       const float2 aux = k.atab[jm];
       const float u0 = su[threadIdx.x], u1 = su[threadIdx.x + 1];
@@ -1052,12 +1130,10 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   g_create_err.clear();
   if (d->mx < 8 || d->mz < 6 || d->tpml < 1 || d->tpml > 256 || d->jmnum < 1 || d->mt2 < 1)
     return create_fail(-2, "need mx >= 8, mz >= 6, 1 <= tpml <= 256, jmnum >= 1, mt2 >= 1");
-  if (!d->stress_imaging)
-    return create_fail(-3, "stress_imaging = 0 (explicit free surface) is not supported");
   if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
       d->lin_x > d->mx - 1 + d->tpml)
     return create_fail(-4, "source cell outside lin_x in [2-tpml, mx-1+tpml], lin_z in [2, mz-2]");
-  if (d->reserved != 0) return create_fail(-1, "reserved must be 0");
+  if (d->variant & ~(BQ_AFDA_ZZ | BQ_AFDA_XZ)) return create_fail(-1, "unknown variant bits");
   const long long nx = (long long)d->mx + 2LL * d->tpml;
   const long long nz = (long long)d->mz + d->tpml + 1;
   const long long pitch = (nx + 63) / 64 * 64;
@@ -1078,6 +1154,9 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   KParams &k = h->kp;
   k.MX = g.MX; k.MZ = g.MZ; k.T = g.T; k.PMLL = g.PMLL; k.PMLZH = g.PMLZH; k.NX = g.NX;
   k.P = h->P; k.OFF = -g.PMLL; k.LIN_X = g.LIN_X; k.LIN_Z = g.LIN_Z; k.MR = 0;
+  k.SI = d->stress_imaging != 0;
+  k.AFDA_ZZ = (d->variant & BQ_AFDA_ZZ) != 0;
+  k.AFDA_XZ = (d->variant & BQ_AFDA_XZ) != 0;
   // MODULE AUXIL constants (mod_auxil.F90:19-20) and AUX1 (aux1.F90:44-46)
   k.DT = g.DT;
   k.A = -1.0f / 24.0f;
@@ -1156,9 +1235,12 @@ int bq_create(const bq_desc *d, bq_handle **out) {
     bq_destroy(h);
     return -100;
   }
-  // cells of the stress half-step from row 2 (rows 0, 1 are its images), of
-  // the velocity half-step from row 2 (row 1 is k_edges')
-  const bq::Plan plan = bq::plan_cells(g, k.OFF, kFW, kFR, kRun, 2, 2);
+  // with stress imaging the cells of both half-steps start at row 2: rows 0, 1
+  // of the stresses are images written by rows 2-4, row 1 of the velocities is
+  // k_edges', row 0 of the velocities is never updated; without it every row
+  // from 0 is a per-cell row
+  const int z0 = k.SI ? 2 : 0;
+  const bq::Plan plan = bq::plan_cells(g, k.OFF, kFW, kFR, kRun, z0, z0);
   h->fu = plan.fu;
   h->ss = plan.ss;
   h->sv = plan.sv;

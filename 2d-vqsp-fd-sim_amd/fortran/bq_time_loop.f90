@@ -65,7 +65,15 @@ SUBROUTINE  TIME_LOOP
   D%KTBO = KTBO
   D%MT2 = MT2
   D%JMNUM = JMNUM
-  D%RESERVED = 0
+  ! the explicit surface's one-sided stencils, as this build of the driver
+  ! was configured (the reference's -DAFDA_ZZ / -DAFDA_XZ)
+  D%VARIANT = 0
+#if(AFDA_ZZ)
+  D%VARIANT = IOR(D%VARIANT, BQ_AFDA_ZZ)
+#endif
+#if(AFDA_XZ)
+  D%VARIANT = IOR(D%VARIANT, BQ_AFDA_XZ)
+#endif
   D%H = REAL(H, C_DOUBLE)
   D%DT = REAL(DT, C_DOUBLE)
   RC = BQ_CREATE(D, HB)

@@ -41,14 +41,20 @@ typedef struct bq_desc {
   int32_t mx;             /* MX: physical cells in x */
   int32_t mz;             /* MZ' : MZ after read_open.F90:149-151 */
   int32_t tpml;           /* TPML (1..256) */
-  int32_t stress_imaging; /* STRESS_IMAGING (logical as 0/1); 1 is supported */
+  int32_t stress_imaging; /* STRESS_IMAGING (logical as 0/1) */
   int32_t lin_x, lin_z;   /* source cell, read_open.F90:163-169 */
   int32_t ktbo;           /* KTBO: bottom boundary key (0 = PML only) */
   int32_t mt2;            /* MT2: number of time steps */
   int32_t jmnum;          /* JMNUM: entries in the material table */
-  int32_t reserved;       /* must be 0 */
+  int32_t variant;        /* BQ_AFDA_* bits: the reference's compile-time options */
   double h, dt;           /* H, DT (binary32 values widened to double) */
 } bq_desc;
+
+/* bq_desc.variant: one-sided stencils of the explicit free surface
+ * (stress_imaging = 0), built into the reference with -DAFDA_ZZ / -DAFDA_XZ:
+ * STRAIN_UW_ZZ_0 / STRAIN_Q_ZZ_0 (strain_uw_zz_0.F90:52-61,
+ * strain_q_zz_0.F90:47-56) and STRAIN_UW_XZ_1 (strain_uw_xz_1.F90:83-96). */
+enum { BQ_AFDA_ZZ = 1, BQ_AFDA_XZ = 2 };
 
 /* Field ids for bq_get_field / bq_set_field (mod_wavefield.F90:20,
  * mod_stressfield.F90:8). */
