@@ -158,6 +158,11 @@ class Job:
     ktle: int = 0
     ktri: int = 0
     omg: float = 0.0
+    # NONREF namelist parameters of the Higdon / optimal-absorption boundary
+    # coefficients (read_open.F90:173-179 defaults)
+    wb: float = 0.4
+    thpple: float = 0.19757
+    thssle: float = 0.70511
     text: str = "biotfd synthetic model"
     extra: dict = field(default_factory=dict)
 
@@ -217,7 +222,8 @@ def write_job(job: Job, directory: str, real_dtype=np.float32) -> None:
             X_SOURCE=job.x_source, Z_SOURCE=job.z_source, KEY_SOUR=job.key_sour,
             QR=job.qr)))
         f.write(format_namelist("NONREF", dict(
-            OMG=job.omg, KTLE=job.ktle, KTRI=job.ktri, KTBO=job.ktbo)))
+            OMG=job.omg, WB=job.wb, THPPLE=job.thpple, THSSLE=job.thssle, KTLE=job.ktle,
+            KTRI=job.ktri, KTBO=job.ktbo)))
         f.write(format_namelist("TXT", dict(TEXT=job.text[:40])))
         f.write(format_namelist("REC", dict(MR=len(job.receivers))))
 

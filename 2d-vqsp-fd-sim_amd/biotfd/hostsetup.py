@@ -71,3 +71,12 @@ def read_stf_reals(job: Job, dtype=np.float32):
     """``STF.DAT`` samples in the working precision."""
     return (np.asarray(job.stf_t, dtype=np.float32).astype(dtype),
             np.asarray(job.stf_v, dtype=np.float32).astype(dtype))
+
+
+def nr_params(job: Job, dtype=np.float32) -> np.ndarray:
+    """OMG, WB, THPPLE, THSSLE as MODULE CONTROL_DATA holds them after
+    READ_OPEN: the namelist OMG divided by the default-real literal 2 pi
+    (read_open.F90:188), in the working precision."""
+    two_pi = dtype(np.float32(6.283185307179586476925286766559))
+    omg = dtype(job.omg) / two_pi
+    return np.array([omg, job.wb, job.thpple, job.thssle], dtype=dtype)

@@ -23,7 +23,7 @@ EXPORTS = (
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells", "bq_probe_copy",
-    "bq_probe_march",
+    "bq_probe_march", "bq_build_bottom_nr", "bq_nonref",
 )
 
 
@@ -75,6 +75,9 @@ def load(path: str = LIBPATH):
         "bq_probe_copy": ([C.c_longlong, C.c_int, p], C.c_int),
         "bq_kernel_cells": ([p, p], C.c_int),
         "bq_probe_march": ([C.c_int, C.c_int, C.c_int, p], C.c_int),
+        "bq_build_bottom_nr": ([p, p, p, p], C.c_int),
+        "bq_nonref": ([C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, p, p],
+                      C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -85,6 +88,18 @@ def load(path: str = LIBPATH):
         f.restype = res
     _lib = lib
     return lib
+
+
+def nonref(key: int, h, dt, vel, vels, velp, prm) -> np.ndarray:
+    """NONREF (nonref.F90:56-206) for one boundary: AX01 AX02 AX10 AX11 AX12
+    AX20 AX21 AX22, computed by the library's host code."""
+    prm = np.ascontiguousarray(prm, np.float32)
+    out = np.zeros(8, np.float32)
+    rc = load().bq_nonref(int(key), float(h), float(dt), float(vel), float(vels), float(velp),
+                          _ptr(prm), _ptr(out))
+    if rc:
+        raise BiotError(f"bq_nonref failed ({rc})")
+    return out
 
 
 def probe_march(nx: int, nz: int, iters: int = 20) -> float:
@@ -150,9 +165,17 @@ class Simulation:
             arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in pml_arrays]
             ptrs = (C.c_void_p * 24)(*[a.ctypes.data for a in arrs])
             self._chk(self.lib.bq_upload_pml(h, ptrs))
-        abw = np.zeros(8, np.float32) if abw is None else np.asarray(abw, np.float32)
-        abh = np.zeros(8, np.float32) if abh is None else np.asarray(abh, np.float32)
-        self._chk(self.lib.bq_upload_bottom_nr(h, _ptr(abw), _ptr(abh)))
+        if abw is None and abh is None and job.ktbo != 0:
+            # PRE_NR_COEF1's bottom coefficients, computed by the library
+            abw = np.zeros(8, np.float32)
+            abh = np.zeros(8, np.float32)
+            prm = hostsetup.nr_params(job, np.float32)
+            self._chk(self.lib.bq_build_bottom_nr(h, _ptr(prm), _ptr(abw), _ptr(abh)))
+        else:
+            abw = np.zeros(8, np.float32) if abw is None else np.asarray(abw, np.float32)
+            abh = np.zeros(8, np.float32) if abh is None else np.asarray(abh, np.float32)
+            self._chk(self.lib.bq_upload_bottom_nr(h, _ptr(abw), _ptr(abh)))
+        self.abw, self.abh = abw, abh
         self.sourtf = source_from_stf(job) if sourtf is None else np.ascontiguousarray(sourtf, np.float32)
         self._chk(self.lib.bq_set_source(h, _ptr(self.sourtf)))
         rec = np.asarray(job.receivers, dtype=np.int32).reshape(-1, 4)

@@ -1387,6 +1387,22 @@ int bq_upload_bottom_nr(bq_handle *h, const float abw[8], const float abh[8]) {
   return 0;
 }
 
+int bq_build_bottom_nr(bq_handle *h, const float prm[4], float abw[8], float abh[8]) {
+  if (!h || !prm) return fail(h, -1, "null argument");
+  if (!h->model_ok) return fail(h, -6, "bq_build_bottom_nr needs bq_upload_model first");
+  const bq::NrParams p{prm[0], prm[1], prm[2], prm[3]};
+  float w[8], hh[8];
+  if (bq::bottom_nr(h->g, h->jm_host.data(), h->tab_host.data(), p, w, hh))
+    return fail(h, -2, "unknown bottom boundary key KTBO");
+  for (int c = 0; c < 8; ++c) {
+    h->kp.abw[c] = w[c];
+    h->kp.abh[c] = hh[c];
+    if (abw) abw[c] = w[c];
+    if (abh) abh[c] = hh[c];
+  }
+  return 0;
+}
+
 int bq_set_source(bq_handle *h, const float *sourtf) {
   if (!h || !sourtf) return fail(h, -1, "null argument");
   const int n = h->g.MT2;

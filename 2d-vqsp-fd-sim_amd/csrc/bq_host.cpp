@@ -193,6 +193,194 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   return p;
 }
 
+// --------------------------------------------- bottom non-reflecting boundary
+//
+// The coefficient generators NONREF calls compute in binary64 from binary32
+// arguments; the products of binary32 arguments they form first (GP, GS of
+// HIGCE_FD, OMG*DT of the optimal boundary) round in binary32, as in the
+// reference. The nine results a00 .. a22 are returned in a[0..8].
+
+// HIGDON_FD, higdon_fd.F90:1-46: product of two first-order Higdon operators
+// with speeds ALFA and BETA.
+static void higdon(float WB, float ALFA, float BETA, float DT, float H, double a[9]) {
+  const double be1 = 1.0;
+  const double be2 = (double)ALFA / (double)BETA;
+  const double ni = (double)ALFA * (double)DT / (double)H;
+  const double qx = (WB * (be1 + ni) - ni) / (be1 + ni) / (1.0 - WB);
+  const double rx = (WB * (be2 + ni) - ni) / (be2 + ni) / (1.0 - WB);
+  const double qt = (WB * (be1 + ni) - be1) / (be1 + ni) / (1.0 - WB);
+  const double rt = (WB * (be2 + ni) - be1) / (be2 + ni) / (1.0 - WB);
+  const double qxt = WB / (WB - 1.0), rxt = WB / (WB - 1.0);
+  a[0] = 0.0;
+  a[1] = -(qx + rx);
+  a[2] = -(qx * rx);
+  a[3] = -(qt + rt);
+  a[4] = -(qx * rt + qt * rx + qxt + rxt);
+  a[5] = -(qx * rxt + rx * qxt);
+  a[6] = -(qt * rt);
+  a[7] = -(qt * rxt + rt * qxt);
+  a[8] = -(qxt * rxt);
+}
+
+// HIGCE_FD, higdce_fd.F90:1-42: Higdon operator with speed BETA times a
+// first-order Clayton-Engquist one with speed ALFA.
+static void higdon_ce(float WB, float ALFA, float BETA, float DT, float H, double a[9]) {
+  const double gp = (float)(ALFA * DT / H);
+  const double gs = (float)(BETA * DT / H);
+  const double gs1 = 1.0 + gs;
+  const float omw = 1.0f - WB;
+  const double h1x = (gs - WB * gs1) / (gs1 * omw);
+  const double h1t = (1.0 - WB * gs1) / (gs1 * omw);
+  const double hxt = WB / (1.0 - WB);
+  const double c1t = 1.0 - gp, cxt = gp;
+  a[0] = 0.0;
+  a[1] = h1x;
+  a[2] = 0.0;
+  a[3] = c1t + h1t;
+  a[4] = cxt + hxt - h1x * c1t;
+  a[5] = -(h1x * cxt);
+  a[6] = -(h1t * c1t);
+  a[7] = -(hxt * c1t) - h1t * cxt;
+  a[8] = -(hxt * cxt);
+}
+
+// HIGCE2_FD, higdce2_fd.F90:1-45: Higdon operator with speed VELH times a
+// Clayton-Engquist operator with speed VELCE.
+static void higdon_ce2(float WB, float VELH, float VELCE, float DT, float H, double a[9]) {
+  const double ni = (double)VELH * (double)DT / (double)H;
+  const double gce = (double)VELCE * (double)DT / (double)H;
+  const double qx = (WB * (1.0 + ni) - ni) / (1.0 + ni) / (1.0 - WB);
+  const double rx = (1.0 - gce) / (1.0 + gce);
+  const double qt = (WB * (1.0 + ni) - 1.0) / (1.0 + ni) / (1.0 - WB);
+  const double rt = -((1.0 - gce) / (1.0 + gce));
+  const double qxt = WB / (WB - 1.0), rxt = -1.0;
+  a[0] = 0.0;
+  a[1] = -(qx + rx);
+  a[2] = -(qx * rx);
+  a[3] = -(qt + rt);
+  a[4] = -(qx * rt + qt * rx + qxt + rxt);
+  a[5] = -(qx * rxt + rx * qxt);
+  a[6] = -(qt * rt);
+  a[7] = -(qt * rxt + rt * qxt);
+  a[8] = -(qxt * rxt);
+}
+
+// OptAbsBc, OptAbsBc.F90:1-106: the 3x3 boundary stencil that minimises the
+// reflection of plane waves at the two incidence angles gp and gm, for the
+// normalised frequency wt = OMG*DT and wavenumber wx = OMG/VEL*H. r0 and r1
+// are the two stencils that each annihilate one angle; the result is
+// r0 + ratio*r1 with ratio the least-squares weight -<r0,r1>/<r1,r1>.
+static void optimal_boundary(double gp, double gm, double wt, double wx, double a[9]) {
+  const double tp = wx * std::cos(gp), tm = wx * std::cos(gm);
+  const double rz = std::cos(wt), iz = std::sin(wt);
+  const double rx = std::cos(tp + tm), ix = std::sin(tp + tm);
+  const double ry = -(std::cos(tp) + std::cos(tm)), iy = -(std::sin(tp) + std::sin(tm));
+  const double p1 = rx * rz - ix * iz, p2 = rx * iz + rz * ix;
+  const double p3 = ry * rz - iy * iz, p4 = ry * iz + rz * iy;
+  double inv = (ry + iy + 2.0 * rz) * (p1 + p2 - rz + iz) + (1.0 - rx - ix) * (2.0 + p3 + p4);
+  inv = 1.0 / inv;
+  double r1[3][3], r0[3][3];
+  r1[1][0] = (1.0 - rz) * (p1 + p2 - rz + iz) * inv;
+  r1[0][2] = (1.0 - rz) * (1.0 - rx - ix) * inv;
+  r1[0][1] = -0.5 - r1[1][0];
+  r1[1][1] = 1.0 - 2.0 * r1[0][2];
+  r1[0][0] = r1[2][2] = 0.0;
+  r1[1][2] = r1[1][0];
+  r1[2][0] = r1[0][2];
+  r1[2][1] = r1[0][1];
+  const double m5 = rx * rz + ix * iz, m6 = rx * iz - rz * ix;
+  const double m7 = ry * rz + iy * iz, m8 = ry * iz - rz * iy;
+  const double phi = -((rz + iz - m5 + m6) * (ry + iy + 2.0 * rz)) + (1.0 + rz + m7 - m8) * (1.0 - rx - ix);
+  const double psi = (1.0 + rz + m7 - m8) * (p1 + p2 - rz + iz) + (rz + iz - m5 + m6) * (2.0 + p3 + p4);
+  r0[0][2] = phi * inv;
+  r0[1][0] = psi * inv;
+  r0[0][0] = 0.0;
+  r0[0][1] = 0.5 - r0[1][0];
+  r0[1][1] = 1.0 - 2.0 * r0[0][2];
+  r0[2][2] = -1.0;
+  r0[1][2] = r0[1][0];
+  r0[2][0] = r0[0][2];
+  r0[2][1] = r0[0][1];
+  double dot = 0.0, nrm = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      dot = dot + r0[i][j] * r1[i][j];
+      nrm = nrm + r1[i][j] * r1[i][j];
+    }
+  const double ratio = -(dot / nrm);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[3 * i + j] = r0[i][j] + ratio * r1[i][j];
+  a[0] = 0.0;
+}
+
+int nonref(int key, float H, float DT, float VEL, float VELS, float VELP, const NrParams &p,
+           float ax[8]) {
+  double a[9];
+  for (int k = 0; k < 8; ++k) ax[k] = 0.0f;
+  switch (key) {
+    case 0:
+      return 0;
+    case 1:
+      higdon(p.WB, VELP, VELS, DT, H, a);
+      break;
+    case 2: {  // first-order one-way wave equation, upwind
+      const float c = -(DT * VEL / H);
+      ax[2] = 1.0f + c;
+      ax[3] = 1.0f - c;
+      ax[6] = -1.0f - c;
+      ax[7] = c;
+      return 0;
+    }
+    case 3:
+    case 9: {  // both keys take the left-boundary angles THPPLE, THSSLE
+      const float wx = key == 3 ? p.OMG / VEL * H : p.OMG / VELS * H;
+      optimal_boundary((double)p.THPPLE, (double)p.THSSLE, (double)(p.OMG * DT), (double)wx, a);
+      break;
+    }
+    case 4: {
+      const float t1 = (DT - H / VEL) / (DT + H / VEL);
+      const float t2 = 2.0f * H / VEL / (DT + H / VEL);
+      ax[0] = t1;
+      ax[2] = t2;
+      ax[3] = t2;
+      ax[5] = t1;
+      ax[6] = -1.0f;
+      return 0;
+    }
+    case 5: {
+      const float c = VEL * DT / H;
+      ax[0] = -(1.0f - c) / (1.0f + c);
+      ax[2] = (1.0f - c) / (1.0f + c);
+      ax[3] = 1.0f;
+      return 0;
+    }
+    case 6:
+      higdon_ce(p.WB, VELP, VELS, DT, H, a);
+      break;
+    case 7:
+      higdon_ce2(p.WB, VEL, VEL, DT, H, a);
+      break;
+    case 8:
+      higdon_ce2(p.WB, VELS, VELP, DT, H, a);
+      break;
+    default:
+      return -1;
+  }
+  for (int k = 0; k < 8; ++k) ax[k] = (float)a[k + 1];
+  return 0;
+}
+
+int bottom_nr(const GridDesc &g, const int32_t *jm, const float *tab, const NrParams &p,
+              float abw[8], float abh[8]) {
+  const int m = jm[(size_t)g.MZ * g.NX] - 1;  // JM(PMLL, MZ')
+  auto tv = [&](int c) { return tab[(size_t)m * T_NTAB + c]; };
+  float vpf, vps, vs;
+  wave_speeds(tv(T_SXX1), -tv(T_P1), -tv(T_P3), tv(T_MU), g.H * tv(T_REL1_U) / g.DT,
+              g.H * tv(T_REL2_QX) / g.DT, g.H * tv(T_REL3_U) / g.DT, &vpf, &vps, &vs);
+  if (nonref(g.KTBO, g.H, g.DT, vpf, vs, vpf, p, abw)) return -1;
+  return nonref(g.KTBO, g.H, g.DT, vs, vs, vpf, p, abh);
+}
+
 }  // namespace bq
 
 extern "C" int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float dt,
@@ -251,4 +439,12 @@ extern "C" int bq_source_from_stf(const float *t, const float *v, int nstf, int 
     }
   }
   return 0;
+}
+
+// NONREF for one boundary, without a handle (host-side set-up helper).
+extern "C" int bq_nonref(int key, float h, float dt, float vel, float vels, float velp,
+                         const float prm[4], float out[8]) {
+  if (!prm || !out) return -1;
+  const bq::NrParams p{prm[0], prm[1], prm[2], prm[3]};
+  return bq::nonref(key, h, dt, vel, vels, velp, p, out) ? -2 : 0;
 }

@@ -73,6 +73,23 @@ struct Plan {
 // fr: fused tile width and height; nr: rows per wavefront run.
 Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv);
 
+// NONREF inputs from MODULE CONTROL_DATA: OMG as READ_OPEN leaves it (the
+// namelist value / 2 pi, read_open.F90:188), WB, THPPLE, THSSLE.
+struct NrParams {
+  float OMG, WB, THPPLE, THSSLE;
+};
+
+// NONREF (nonref.F90:56-206): ax = AX01 AX02 AX10 AX11 AX12 AX20 AX21 AX22 of
+// boundary key `key` for speed VEL (VELS, VELP: S and fast-P speeds). Returns
+// 0, or -1 for an unknown key.
+int nonref(int key, float H, float DT, float VEL, float VELS, float VELP, const NrParams &p,
+           float ax[8]);
+
+// Bottom part of PRE_NR_COEF1 (pre_nr_coef1.F90:185-201): ABW from the fast-P
+// speed, ABH from the S speed of the material at (PMLL, MZ').
+int bottom_nr(const GridDesc &g, const int32_t *jm, const float *tab, const NrParams &p,
+              float abw[8], float abh[8]);
+
 void wave_speeds(float sig1, float p, float ainv, float mu, float rel1, float rel2, float rel3,
                  float *vpf, float *vps, float *vs);
 void build_pml(const GridDesc &g, const int32_t *jm, const float *tab, float r, float tau,

@@ -96,6 +96,15 @@ int bq_get_pml(bq_handle *h, int id, float *out);
  * order 01 02 10 11 12 20 21 22. All zero for KTBO = 0. */
 int bq_upload_bottom_nr(bq_handle *h, const float abw[8], const float abh[8]);
 
+/* Alternative to bq_upload_bottom_nr: PRE_NR_COEF1's bottom part
+ * (pre_nr_coef1.F90:185-201, NONREF nonref.F90:56-206 with HIGDON_FD,
+ * HIGCE_FD, HIGCE2_FD, OptAbsBc) computed on the host from the uploaded model
+ * for key desc.ktbo (0..9), in the reference's binary32 / binary64 order, and
+ * uploaded. prm = OMG as READ_OPEN leaves it (namelist OMG / 2 pi,
+ * read_open.F90:188), WB, THPPLE, THSSLE (read_open.F90:173-179). abw, abh
+ * receive the coefficients if not NULL. Needs bq_upload_model first. */
+int bq_build_bottom_nr(bq_handle *h, const float prm[4], float abw[8], float abh[8]);
+
 /* SOURTF_A(2,MT2) (open_sr_file.F90:78-110): per step, solid then fluid. */
 int bq_set_source(bq_handle *h, const float *sourtf);
 
@@ -126,6 +135,12 @@ int bq_set_field(bq_handle *h, int field_id, const float *host);
  * STF.DAT samples onto t=(n-1)*DT, scaled by KEY_SOUR/QR/H, -> sourtf[2*mt2]. */
 int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float dt, float h,
                        int key_sour, float qr, float *sourtf);
+
+/* NONREF (nonref.F90:56-206) for one boundary and key 0..9: out = AX01 AX02
+ * AX10 AX11 AX12 AX20 AX21 AX22 for speed vel (vels, velp: the S and fast-P
+ * speeds of the cell), prm as for bq_build_bottom_nr. */
+int bq_nonref(int key, float h, float dt, float vel, float vels, float velp, const float prm[4],
+              float out[8]);
 
 /* Informational. */
 const char *bq_last_error(bq_handle *h);

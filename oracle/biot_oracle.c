@@ -1436,13 +1436,157 @@ EXPORT int oracle_source_from_stf(const real *t, const real *v, int nstf, int mt
  * (nonref.F90:69-204) for the closed-form keys 0, 2, 4 and 5: wave speeds of
  * the cell JM(PMLL, MZ'), ABW from the fast-P speed, ABH from the S speed.
  * Returns -1 for keys whose coefficients come from HIGDON_FD/OptAbsBc. */
-static int nonref_key(int key, real H, real DT, real VEL, real *ax) {
+/* Parameters of MODULE CONTROL_DATA that NONREF reads: OMG as READ_OPEN leaves
+ * it (namelist value / 2 pi, read_open.F90:188), WB, THPPLE, THSSLE
+ * (read_open.F90:173-179 defaults). */
+typedef struct { real OMG, WB, THPPLE, THSSLE; } nr_params;
+
+/* HIGDON_FD, higdon_fd.F90:1-46 (DP arithmetic on PP arguments) */
+static void higdon_fd(real WB, real ALFA, real BETA, real DT, real H, double *a) {
+  double BE1 = 1.0;
+  double BE2 = (double)ALFA / (double)BETA;
+  double NI = (double)ALFA * (double)DT / (double)H;
+  double QX = (WB * (BE1 + NI) - NI) / (BE1 + NI) / (1.0 - WB);
+  double RX = (WB * (BE2 + NI) - NI) / (BE2 + NI) / (1.0 - WB);
+  double QT = (WB * (BE1 + NI) - BE1) / (BE1 + NI) / (1.0 - WB);
+  double RT = (WB * (BE2 + NI) - BE1) / (BE2 + NI) / (1.0 - WB);
+  double QXT = WB / (WB - 1.0);
+  double RXT = WB / (WB - 1.0);
+  a[0] = 0.0;                                  /* A00 */
+  a[1] = -(QX + RX);                           /* A01 */
+  a[2] = -(QX * RX);                           /* A02 */
+  a[3] = -(QT + RT);                           /* A10 */
+  a[4] = -(QX * RT + QT * RX + QXT + RXT);     /* A11 */
+  a[5] = -(QX * RXT + RX * QXT);               /* A12 */
+  a[6] = -(QT * RT);                           /* A20 */
+  a[7] = -(QT * RXT + RT * QXT);               /* A21 */
+  a[8] = -(QXT * RXT);                         /* A22 */
+}
+
+/* HIGCE_FD, higdce_fd.F90:1-42: GP, GS are formed in PP, and so is (1-WB) */
+static void higce_fd(real WB, real ALFA, real BETA, real DT, real H, double *a) {
+  double GP = (real)(ALFA * DT / H);
+  double GS = (real)(BETA * DT / H);
+  double GS1 = 1.0 + GS;
+  real OMW = (real)1 - WB;
+  double H1X = (GS - WB * GS1) / (GS1 * OMW);
+  double H1T = (1.0 - WB * GS1) / (GS1 * OMW);
+  double HXT = WB / (1.0 - WB);
+  double C1T = 1.0 - GP;
+  double CXT = GP;
+  a[0] = 0.0;
+  a[1] = H1X;
+  a[2] = 0.0;
+  a[3] = C1T + H1T;
+  a[4] = CXT + HXT - H1X * C1T;
+  a[5] = -(H1X * CXT);
+  a[6] = -(H1T * C1T);
+  a[7] = -(HXT * C1T) - H1T * CXT;
+  a[8] = -(HXT * CXT);
+}
+
+/* HIGCE2_FD, higdce2_fd.F90:1-45 */
+static void higce2_fd(real WB, real VELH, real VELCE, real DT, real H, double *a) {
+  double NI = (double)VELH * (double)DT / (double)H;
+  double G = (double)VELCE * (double)DT / (double)H;
+  double QX = (WB * (1.0 + NI) - NI) / (1.0 + NI) / (1.0 - WB);
+  double RX = (1.0 - G) / (1.0 + G);
+  double QT = (WB * (1.0 + NI) - 1.0) / (1.0 + NI) / (1.0 - WB);
+  double RT = -((1.0 - G) / (1.0 + G));
+  double QXT = WB / (WB - 1.0);
+  double RXT = -1.0;
+  a[0] = 0.0;
+  a[1] = -(QX + RX);
+  a[2] = -(QX * RX);
+  a[3] = -(QT + RT);
+  a[4] = -(QX * RT + QT * RX + QXT + RXT);
+  a[5] = -(QX * RXT + RX * QXT);
+  a[6] = -(QT * RT);
+  a[7] = -(QT * RXT + RT * QXT);
+  a[8] = -(QXT * RXT);
+}
+
+/* OptAbsBc, OptAbsBc.F90:1-106 (DP; the literals are exact binary values) */
+static void opt_abs_bc(double gplus, double gminus, double dt, double dx, double *a) {
+  double tplus = dx * cos(gplus);
+  double tminus = dx * cos(gminus);
+  double Rz = cos(dt), Iz = sin(dt);
+  double Rx = cos(tplus + tminus), Ix = sin(tplus + tminus);
+  double Ry = -(cos(tplus) + cos(tminus));
+  double Iy = -(sin(tplus) + sin(tminus));
+  double temp1 = Rx * Rz - Ix * Iz;
+  double temp2 = Rx * Iz + Rz * Ix;
+  double temp3 = Ry * Rz - Iy * Iz;
+  double temp4 = Ry * Iz + Rz * Iy;
+  double delta = (Ry + Iy + 2.0 * Rz) * (temp1 + temp2 - Rz + Iz) +
+                 (1.0 - Rx - Ix) * (2.0 + temp3 + temp4);
+  delta = 1.0 / delta;
+  double r1[3][3], r0[3][3];
+  r1[1][0] = (1.0 - Rz) * (temp1 + temp2 - Rz + Iz) * delta;
+  r1[0][2] = (1.0 - Rz) * (1.0 - Rx - Ix) * delta;
+  r1[0][1] = -0.50 - r1[1][0];
+  r1[1][1] = 1.0 - 2.0 * r1[0][2];
+  r1[0][0] = 0.0;
+  r1[2][2] = 0.0;
+  r1[1][2] = r1[1][0];
+  r1[2][0] = r1[0][2];
+  r1[2][1] = r1[0][1];
+  double temp5 = Rx * Rz + Ix * Iz;
+  double temp6 = Rx * Iz - Rz * Ix;
+  double temp7 = Ry * Rz + Iy * Iz;
+  double temp8 = Ry * Iz - Rz * Iy;
+  double phi = -((Rz + Iz - temp5 + temp6) * (Ry + Iy + 2.0 * Rz)) +
+               (1.0 + Rz + temp7 - temp8) * (1.0 - Rx - Ix);
+  r0[0][2] = phi * delta;
+  double psi = (1.0 + Rz + temp7 - temp8) * (temp1 + temp2 - Rz + Iz) +
+               (Rz + Iz - temp5 + temp6) * (2.0 + temp3 + temp4);
+  r0[1][0] = psi * delta;
+  r0[0][0] = 0.;
+  r0[0][1] = 0.50 - r0[1][0];
+  r0[1][1] = 1.0 - 2.0 * r0[0][2];
+  r0[2][2] = -1.0;
+  r0[1][2] = r0[1][0];
+  r0[2][0] = r0[0][2];
+  r0[2][1] = r0[0][1];
+  temp1 = 0.0;
+  temp2 = 0.0;
+  for (int i = 0; i <= 2; ++i)
+    for (int j = 0; j <= 2; ++j) {
+      temp1 = temp1 + r0[i][j] * r1[i][j];
+      temp2 = temp2 + r1[i][j] * r1[i][j];
+    }
+  double ratio = -(temp1 / temp2);
+  a[0] = 0.0;
+  a[1] = r0[0][1] + ratio * r1[0][1];
+  a[2] = r0[0][2] + ratio * r1[0][2];
+  a[3] = r0[1][0] + ratio * r1[1][0];
+  a[4] = r0[1][1] + ratio * r1[1][1];
+  a[5] = r0[1][2] + ratio * r1[1][2];
+  a[6] = r0[2][0] + ratio * r1[2][0];
+  a[7] = r0[2][1] + ratio * r1[2][1];
+  a[8] = r0[2][2] + ratio * r1[2][2];
+}
+
+/* NONREF, nonref.F90:56-206: ax = AX01, AX02, AX10, AX11, AX12, AX20, AX21, AX22.
+ * Keys 3 and 9 read THPPLE and THSSLE whichever boundary is served, and hand
+ * back REAL(aa) - default (binary32) reals. */
+static int nonref_key(int key, real H, real DT, real VEL, real VELS, real VELP,
+                      const nr_params *p, real *ax) {
+  double aa[9];
   for (int k = 0; k < 8; ++k) ax[k] = 0;
   switch (key) {
     case 0: return 0;
+    case 1: higdon_fd(p->WB, VELP, VELS, DT, H, aa); break;
     case 2: {
-      real TMP = -DT * VEL / H;
+      real TMP = -(DT * VEL / H);
       ax[2] = (real)1 + TMP; ax[3] = (real)1 - TMP; ax[6] = (real)-1 - TMP; ax[7] = TMP;
+      return 0;
+    }
+    case 3:
+    case 9: {
+      real HX = key == 3 ? p->OMG / VEL * H : p->OMG / VELS * H;
+      opt_abs_bc((double)p->THPPLE, (double)p->THSSLE, (double)(real)(p->OMG * DT), (double)HX, aa);
+      for (int k = 0; k < 8; ++k) ax[k] = (real)(float)aa[k + 1];
       return 0;
     }
     case 4: {
@@ -1458,17 +1602,33 @@ static int nonref_key(int key, real H, real DT, real VEL, real *ax) {
       ax[3] = (real)1;
       return 0;
     }
+    case 6: higce_fd(p->WB, VELP, VELS, DT, H, aa); break;
+    case 7: higce2_fd(p->WB, VEL, VEL, DT, H, aa); break;
+    case 8: higce2_fd(p->WB, VELS, VELP, DT, H, aa); break;
     default: return -1;
   }
+  for (int k = 0; k < 8; ++k) ax[k] = (real)aa[k + 1];
+  return 0;
 }
 
-EXPORT int oracle_bottom_nr(orc *o, int key, real *abw, real *abh) {
+/* PRE_NR_COEF1 bottom part, pre_nr_coef1.F90:185-201: ABW from the fast P
+ * speed, ABH from the S speed of the material at (X_MIN, MZ'). prm = OMG (as
+ * READ_OPEN leaves it), WB, THPPLE, THSSLE. */
+EXPORT int oracle_bottom_nr(orc *o, int key, const real *prm, real *abw, real *abh) {
   int jm = JMF(o->PMLL, o->MZ);
   real VPF, VPS, VS;
   calcul_velocity(TAB(T_SXX1, jm), -TAB(T_P1, jm), -TAB(T_P3, jm), TAB(T_MU, jm),
                   o->H * TAB(T_REL1_U, jm) / o->DT, o->H * TAB(T_REL2_QX, jm) / o->DT,
                   o->H * TAB(T_REL3_U, jm) / o->DT, &VPF, &VPS, &VS);
-  if (nonref_key(key, o->H, o->DT, VPF, abw)) return -1;
-  if (nonref_key(key, o->H, o->DT, VS, abh)) return -1;
+  nr_params p = {prm[0], prm[1], prm[2], prm[3]};
+  if (nonref_key(key, o->H, o->DT, VPF, VS, VPF, &p, abw)) return -1;
+  if (nonref_key(key, o->H, o->DT, VS, VS, VPF, &p, abh)) return -1;
   return 0;
+}
+
+/* NONREF alone, for tests of the library's restatement. */
+EXPORT int oracle_nonref(int key, real h, real dt, real vel, real vels, real velp, const real *prm,
+                         real *ax) {
+  nr_params p = {prm[0], prm[1], prm[2], prm[3]};
+  return nonref_key(key, h, dt, vel, vels, velp, &p, ax);
 }

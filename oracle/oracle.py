@@ -58,13 +58,16 @@ def load(precision: str = "f32"):
         ("oracle_set_field", [p, C.c_int, p]),
         ("oracle_get_traces", [p, p, C.c_int, C.c_int]),
         ("oracle_get_pml", [p, C.c_int, p]),
-        ("oracle_bottom_nr", [p, C.c_int, p, p]),
+        ("oracle_bottom_nr", [p, C.c_int, p, p, p]),
         ("oracle_source_from_stf", [p, p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                     C.c_double, p]),
     ):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = None if name == "oracle_destroy" else C.c_int
+    real = C.c_float if precision == "f32" else C.c_double
+    lib.oracle_nonref.argtypes = [C.c_int, real, real, real, real, real, p, p]
+    lib.oracle_nonref.restype = C.c_int
     lib.oracle_last_error.argtypes = [p]
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_real_size.restype = C.c_int
@@ -111,7 +114,8 @@ class OracleSim:
         if abw is None and abh is None and job.ktbo != 0:
             abw = np.zeros(8, self.dtype)
             abh = np.zeros(8, self.dtype)
-            self._chk(self.lib.oracle_bottom_nr(h, job.ktbo, _ptr(abw), _ptr(abh)))
+            prm = hostsetup.nr_params(job, self.dtype)
+            self._chk(self.lib.oracle_bottom_nr(h, job.ktbo, _ptr(prm), _ptr(abw), _ptr(abh)))
         abw = np.zeros(8, self.dtype) if abw is None else np.asarray(abw, self.dtype)
         abh = np.zeros(8, self.dtype) if abh is None else np.asarray(abh, self.dtype)
         self.abw, self.abh = abw, abh
@@ -164,3 +168,16 @@ class OracleSim:
             self.close()
         except Exception:
             pass
+
+
+def nonref(key, h, dt, vel, vels, velp, prm, precision="f32"):
+    """NONREF (nonref.F90:56-206) for one boundary: the eight coefficients
+    AX01, AX02, AX10, AX11, AX12, AX20, AX21, AX22."""
+    lib = load(precision)
+    dtype = np.float32 if precision == "f32" else np.float64
+    prm = np.ascontiguousarray(prm, dtype)
+    out = np.zeros(8, dtype)
+    rc = lib.oracle_nonref(int(key), h, dt, vel, vels, velp, _ptr(prm), _ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle_nonref failed ({rc})")
+    return out

@@ -38,7 +38,8 @@ REF = os.path.join(ROOT, "oracle", "_ref", "fd2d")
 REF_FMA = os.path.join(ROOT, "oracle", "_ref", "fd2d_fma")
 
 SCALARS = ("name", "mx", "mz", "h", "dt", "mt", "x_source", "z_source", "tpml", "stress_imaging",
-           "afda", "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo")
+           "afda", "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo", "omg", "wb",
+           "thpple", "thssle")
 
 
 def cases():
@@ -61,6 +62,17 @@ def cases():
     k.key_sour = 2
     k.qr = 0.3
     out["homog_fluid_source"] = k
+    # the other bottom boundary keys of NONREF (nonref.F90:56-206), source near
+    # the bottom so that the boundary row shapes the snapshots early; keys 3 and
+    # 9 also read OMG (here 40 after READ_OPEN's division by 2 pi)
+    for key in (1, 3, 4, 5, 6, 7, 8, 9):
+        k = jobs.tiny_layered(steps=120, seed=20 + key)
+        k.ktbo = key
+        k.wc = 60.0
+        k.sf = 0.5
+        k.omg = 40.0 * 6.283185307179586
+        k.z_source = (k.mz - 8) * k.h
+        out[f"ktbo{key}"] = k
     return out
 
 
@@ -96,6 +108,8 @@ def main():
         exe, exe_fma = REF, REF_FMA
         steps = job.mt2
         snap_steps = sorted({1, 2, 10, steps // 2, steps})
+        if name.startswith("ktbo"):
+            snap_steps = [steps // 2, steps]
         with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
             snaps, recs = run(exe, job, d1, snap_steps)
             snaps_fma, _ = run(exe_fma, job, d2, [steps])

@@ -56,3 +56,22 @@ def test_create_rejects_bad_descriptions():
         assert h.value is None
         msg = so.bq_last_error(None).decode()
         assert msg.startswith("bq_create: ") and w in msg, msg
+
+
+@pytest.mark.parametrize("key", range(10))
+def test_nonref_matches_oracle(key):
+    """NONREF restated in the library's host code vs the oracle, bit for bit,
+    over a spread of speeds, cell sizes and steps (the oracle's NONREF is
+    pinned by the reference's KTBO fixtures)."""
+    rng = np.random.default_rng(100 + key)
+    for _ in range(50):
+        h = np.float32(rng.uniform(0.5, 5.0))
+        vp = np.float32(rng.uniform(1500.0, 5000.0))
+        vs = np.float32(vp * rng.uniform(0.3, 0.7))
+        dt = np.float32(0.4 * h / vp)
+        prm = np.array([rng.uniform(5.0, 80.0), rng.uniform(0.2, 0.6), 0.19757, 0.70511], np.float32)
+        for vel in (vp, vs):
+            a = lib.nonref(key, h, dt, vel, vs, vp, prm)
+            b = oracle.nonref(key, h, dt, vel, vs, vp, prm, "f32")
+            np.testing.assert_array_equal(a, b)
+            assert np.isfinite(a).all()

@@ -67,6 +67,19 @@ def test_bottom_nr_and_upload_pml():
     assert np.isfinite(sim.field("U")).all()
 
 
+@pytest.mark.parametrize("key", range(10))
+def test_bottom_nr_coefficients_match_oracle(key):
+    """bq_build_bottom_nr (from the uploaded model) vs the oracle's PRE_NR_COEF1."""
+    job = jobs.tiny_layered(steps=2)
+    job.ktbo = key
+    job.wc = 40.0
+    job.omg = 6.283185307179586 * 30.0
+    ref = oracle.OracleSim(job, "f32")
+    sim = lib.Simulation(job)
+    np.testing.assert_array_equal(sim.abw, ref.abw)
+    np.testing.assert_array_equal(sim.abh, ref.abh)
+
+
 def _golden_names():
     import make_golden
     return make_golden.names()
@@ -78,11 +91,7 @@ def test_bitexact_vs_reference_snapshots(name):
     import make_golden
     job, z = make_golden.load(name)
     print(f"golden case {name}", flush=True)
-    abw = abh = None
-    if job.ktbo:
-        o = oracle.OracleSim(job, "f32")
-        abw, abh = o.abw, o.abh
-    sim = lib.Simulation(job, abw=abw, abh=abh)
+    sim = lib.Simulation(job)   # KTBO != 0: the library computes PRE_NR_COEF1's coefficients
     done = 0
     for k, step in enumerate(z["snap_steps"]):
         sim.run(int(step) - done)
