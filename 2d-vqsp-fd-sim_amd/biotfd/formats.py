@@ -138,9 +138,10 @@ class Job:
     receivers: Sequence[tuple] = ()   # (irecx, lrecx, irecz, lrecz)
     tpml: int = 20
     stress_imaging: bool = True
-    # compile-time options of the reference's non-imaging free surface, as
-    # bits: 1 = AFDA_ZZ (strain_uw_zz_0.F90:52), 2 = AFDA_XZ (strain_uw_xz_1.F90:83)
-    afda: int = 0
+    # the reference's compile-time options, as bits: 1 = AFDA_ZZ
+    # (strain_uw_zz_0.F90:52), 2 = AFDA_XZ (strain_uw_xz_1.F90:83), 4 = SECOND
+    # (mod_auxil.F90:15-21: A = 0, B = 1, second-order stencils)
+    variant: int = 0
     r: float = 0.001
     tau: float = 1.5
     pow: float = 2.0

@@ -150,7 +150,7 @@ class Simulation:
         self.job = job
         self.nz, self.nx = hostsetup.grid_shape(job)
         d = Desc(job.mx, job.mzp, job.tpml, int(job.stress_imaging), job.lin_x, job.lin_z,
-                 job.ktbo, job.mt2, job.table.shape[0], int(job.afda), job.h, float(np.float32(job.dt)))
+                 job.ktbo, job.mt2, job.table.shape[0], int(job.variant), job.h, float(np.float32(job.dt)))
         h = C.c_void_p()
         rc = self.lib.bq_create(C.byref(d), C.byref(h))
         if rc:

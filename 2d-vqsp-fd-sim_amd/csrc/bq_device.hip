@@ -45,6 +45,7 @@ struct KParams {
   int MX, MZ, T, PMLL, PMLZH, NX, P, OFF, LIN_X, LIN_Z, MR;
   int SI;                // stress imaging (else explicit surface rows 0 and 1)
   int AFDA_ZZ, AFDA_XZ;  // one-sided surface options of the explicit surface
+  int SECOND;            // second-order build: A = 0, B = 1
   float DT, A, B, AH, BH, CH, BC;
   // fields of the current step (read) and of the next step (written); the two
   // sets trade places after every step
@@ -290,12 +291,16 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         if (l != 0) {
           sx = A * (FA(k.TXX1, i + 1, l) - FA(k.TXX1, i - 2, l) + FA(k.TXZ1, i, l + 2) - FA(k.TXZ1, i, l - 1)) +
                B * (FA(k.TXX1, i, l) - FA(k.TXX1, i - 1, l) + FA(k.TXZ1, i, l + 1) - FA(k.TXZ1, i, l));
-        } else {  // TXZ(i,0) = 0 on the surface: one-sided z-derivative from rows 1-4
+        } else if (!k.SECOND) {  // TXZ(i,0) = 0 on the surface: one-sided z-derivative from rows 1-4
           sx = A * (FA(k.TXX1, i + 1, 0) - FA(k.TXX1, i - 2, 0)) + B * (FA(k.TXX1, i, 0) - FA(k.TXX1, i - 1, 0)) +
                c17_24 * FA(k.TXZ1, i, 1) + c3_8 * FA(k.TXZ1, i, 2) - c5_24 * FA(k.TXZ1, i, 3) +
                c1_24 * FA(k.TXZ1, i, 4);
+        } else {  // second-order build (inh_qx_int_0.F90:40-47)
+          sx = (FA(k.TXX1, i, 0) - FA(k.TXX1, i - 1, 0)) + FA(k.TXZ1, i, 1);
         }
-        const float px = A * (FA(k.PR1, i + 1, l) - FA(k.PR1, i - 2, l)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i - 1, l));
+        const float px = (l == 0 && k.SECOND)
+                             ? (FA(k.PR1, i, 0) - FA(k.PR1, i - 1, 0))
+                             : A * (FA(k.PR1, i + 1, l) - FA(k.PR1, i - 2, l)) + B * (FA(k.PR1, i, l) - FA(k.PR1, i - 1, l));
         qn = v0.x * q - v0.y * sx - v0.z * px;
         un = u + v0.w * q + v1.x * sx + v1.y * px;
       } else {
@@ -1133,7 +1138,8 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
       d->lin_x > d->mx - 1 + d->tpml)
     return create_fail(-4, "source cell outside lin_x in [2-tpml, mx-1+tpml], lin_z in [2, mz-2]");
-  if (d->variant & ~(BQ_AFDA_ZZ | BQ_AFDA_XZ)) return create_fail(-1, "unknown variant bits");
+  if (d->variant & ~(BQ_AFDA_ZZ | BQ_AFDA_XZ | BQ_SECOND))
+    return create_fail(-1, "unknown variant bits");
   const long long nx = (long long)d->mx + 2LL * d->tpml;
   const long long nz = (long long)d->mz + d->tpml + 1;
   const long long pitch = (nx + 63) / 64 * 64;
@@ -1157,10 +1163,11 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   k.SI = d->stress_imaging != 0;
   k.AFDA_ZZ = (d->variant & BQ_AFDA_ZZ) != 0;
   k.AFDA_XZ = (d->variant & BQ_AFDA_XZ) != 0;
+  k.SECOND = (d->variant & BQ_SECOND) != 0;
   // MODULE AUXIL constants (mod_auxil.F90:19-20) and AUX1 (aux1.F90:44-46)
   k.DT = g.DT;
-  k.A = -1.0f / 24.0f;
-  k.B = 9.0f / 8.0f;
+  k.A = k.SECOND ? 0.0f : -1.0f / 24.0f;  // mod_auxil.F90:15-21
+  k.B = k.SECOND ? 1.0f : 9.0f / 8.0f;
   k.AH = k.A / g.H;
   k.BH = k.B / g.H;
   k.CH = 1.0f / g.H;

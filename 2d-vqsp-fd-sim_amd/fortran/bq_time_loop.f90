@@ -65,14 +65,17 @@ SUBROUTINE  TIME_LOOP
   D%KTBO = KTBO
   D%MT2 = MT2
   D%JMNUM = JMNUM
-  ! the explicit surface's one-sided stencils, as this build of the driver
-  ! was configured (the reference's -DAFDA_ZZ / -DAFDA_XZ)
+  ! the build options this driver was configured with (the reference's
+  ! -DAFDA_ZZ / -DAFDA_XZ one-sided surface stencils, -DSECOND second order)
   D%VARIANT = 0
 #if(AFDA_ZZ)
   D%VARIANT = IOR(D%VARIANT, BQ_AFDA_ZZ)
 #endif
 #if(AFDA_XZ)
   D%VARIANT = IOR(D%VARIANT, BQ_AFDA_XZ)
+#endif
+#if(SECOND)
+  D%VARIANT = IOR(D%VARIANT, BQ_SECOND)
 #endif
   D%H = REAL(H, C_DOUBLE)
   D%DT = REAL(DT, C_DOUBLE)

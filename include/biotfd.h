@@ -50,11 +50,14 @@ typedef struct bq_desc {
   double h, dt;           /* H, DT (binary32 values widened to double) */
 } bq_desc;
 
-/* bq_desc.variant: one-sided stencils of the explicit free surface
- * (stress_imaging = 0), built into the reference with -DAFDA_ZZ / -DAFDA_XZ:
- * STRAIN_UW_ZZ_0 / STRAIN_Q_ZZ_0 (strain_uw_zz_0.F90:52-61,
- * strain_q_zz_0.F90:47-56) and STRAIN_UW_XZ_1 (strain_uw_xz_1.F90:83-96). */
-enum { BQ_AFDA_ZZ = 1, BQ_AFDA_XZ = 2 };
+/* bq_desc.variant: the reference's build options. BQ_AFDA_ZZ, BQ_AFDA_XZ: the
+ * one-sided stencils of the explicit free surface (stress_imaging = 0),
+ * -DAFDA_ZZ / -DAFDA_XZ: STRAIN_UW_ZZ_0 / STRAIN_Q_ZZ_0 (strain_uw_zz_0.F90:52-61,
+ * strain_q_zz_0.F90:47-56) and STRAIN_UW_XZ_1 (strain_uw_xz_1.F90:83-96).
+ * BQ_SECOND: the second-order build -DSECOND, A = 0, B = 1 (mod_auxil.F90:15-21)
+ * with the 2-point row-0 forms of inh_qx_int_0.F90:40-47 and
+ * inh_v_uv_int_0.F90:40-48. */
+enum { BQ_AFDA_ZZ = 1, BQ_AFDA_XZ = 2, BQ_SECOND = 4 };
 
 /* Field ids for bq_get_field / bq_set_field (mod_wavefield.F90:20,
  * mod_stressfield.F90:8). */

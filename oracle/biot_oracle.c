@@ -62,14 +62,16 @@ typedef struct oracle_desc {
   double h, dt;
 } oracle_desc;
 
-/* variant bits: the reference's compile-time options of the non-imaging
- * free surface (strain_uw_zz_0.F90:52, strain_q_zz_0.F90:47, strain_uw_xz_1.F90:83) */
-enum { V_AFDA_ZZ = 1, V_AFDA_XZ = 2 };
+/* variant bits: the reference's compile-time options - the one-sided stencils
+ * of the non-imaging free surface (strain_uw_zz_0.F90:52, strain_q_zz_0.F90:47,
+ * strain_uw_xz_1.F90:83) and the second-order build SECOND (mod_auxil.F90:15-21:
+ * A = 0, B = 1; inh_qx_int_0.F90:40-47, inh_v_uv_int_0.F90:40-48) */
+enum { V_AFDA_ZZ = 1, V_AFDA_XZ = 2, V_SECOND = 4 };
 
 typedef struct {
   /* sizes (aux1.F90:36-46) */
   int MX, MZ, T, PMLL, PMLXH, PMLZH, NX, NZ, MX1;
-  int LIN_X, LIN_Z, SI, AFDA_ZZ, AFDA_XZ, KTBO, MT2, JMNUM, next_step;
+  int LIN_X, LIN_Z, SI, AFDA_ZZ, AFDA_XZ, SECOND, KTBO, MT2, JMNUM, next_step;
   real H, DT, A, B, AH, BH, CH;
   int32_t *JM;
   real *tab[T_N];
@@ -792,11 +794,19 @@ static void hplane_1_velocity_si(orc *o) {
   (((((lead) - (real)31 / (real)24 * F(I, 0)) + (real)29 / (real)24 * F(I, 1)) -               \
     (real)3 / (real)40 * F(I, 2)) + (real)1 / (real)168 * F(I, 3))
 
-/* INH_QX_INT_0, inh_qx_int_0.F90:43-71 (4th-order branch) */
+/* INH_QX_INT_0, inh_qx_int_0.F90:43-71; with SECOND the 2-point form of
+ * :40-47, whose pressure term lacks a closing parenthesis in the reference
+ * (so that build does not compile) and is read as (PRES(I,0) - PRES(I-1,0)) */
 static void inh_qx_int_0(orc *o) {
   const real A = o->A, B = o->B;
   for (int I = 4; I <= o->MX - 2; ++I) {
     int jm = JMF(I, 0);
+    if (o->SECOND) {
+      QXF(I, 0) = rexp(-TAB(T_REL3_QX, jm)) * QXF(I, 0) -
+                  TAB(T_REL1_QX, jm) * ((TXXF(I, 0) - TXXF(I - 1, 0)) + TXZF(I, 1)) -
+                  TAB(T_REL2_QX, jm) * (PRF(I, 0) - PRF(I - 1, 0));
+      continue;
+    }
     QXF(I, 0) = rexp(-TAB(T_REL3_QX, jm)) * QXF(I, 0) -
                 TAB(T_REL1_QX, jm) *
                     TXZ_Z0(A * (TXXF(I + 1, 0) - TXXF(I - 2, 0)) + B * (TXXF(I, 0) - TXXF(I - 1, 0)), I) -
@@ -804,11 +814,20 @@ static void inh_qx_int_0(orc *o) {
   }
 }
 
-/* INH_V_UV_INT_0, inh_v_uv_int_0.F90:43-72 (4th-order branch) */
+/* INH_V_UV_INT_0, inh_v_uv_int_0.F90:43-72; with SECOND the 2-point form of
+ * :40-48 (same missing parenthesis) */
 static void inh_v_uv_int_0(orc *o) {
   const real A = o->A, B = o->B;
   for (int I = 4; I <= o->MX - 2; ++I) {
     int jm = JMF(I, 0);
+    if (o->SECOND) {
+      UM(I, 0) = UM(I, 0) +
+                 (TAB(T_REL2_U, jm) / TAB(T_REL3_QX, jm)) * ((real)1 - rexp(-TAB(T_REL3_QX, jm))) *
+                     QXPF(I, 0) +
+                 TAB(T_REL1_U, jm) * ((TXXF(I, 0) - TXXF(I - 1, 0)) + TXZF(I, 1)) +
+                 TAB(T_REL3_U, jm) * (PRF(I, 0) - PRF(I - 1, 0));
+      continue;
+    }
     UM(I, 0) = UM(I, 0) +
                (TAB(T_REL2_U, jm) / TAB(T_REL3_QX, jm)) * ((real)1 - rexp(-TAB(T_REL3_QX, jm))) *
                    QXPF(I, 0) +
@@ -1173,19 +1192,20 @@ EXPORT int oracle_create(const oracle_desc *d, orc **out) {
   if (!d || !out) return -1;
   *out = NULL;
   if (d->mx < 8 || d->mz < 6 || d->tpml < 1 || d->tpml > 256 || d->jmnum < 1) return -2;
-  if (d->variant & ~(V_AFDA_ZZ | V_AFDA_XZ)) return -3;
+  if (d->variant & ~(V_AFDA_ZZ | V_AFDA_XZ | V_SECOND)) return -3;
   if (d->lin_z < 2 || d->lin_z > d->mz - 2 || d->lin_x < 2 - d->tpml ||
       d->lin_x > d->mx - 1 + d->tpml)
     return -4;
   orc *o = (orc *)zalloc(sizeof(orc));
   o->MX = d->mx; o->MZ = d->mz; o->T = d->tpml; o->SI = d->stress_imaging != 0;
   o->AFDA_ZZ = (d->variant & V_AFDA_ZZ) != 0; o->AFDA_XZ = (d->variant & V_AFDA_XZ) != 0;
+  o->SECOND = (d->variant & V_SECOND) != 0;
   o->LIN_X = d->lin_x; o->LIN_Z = d->lin_z; o->KTBO = d->ktbo; o->MT2 = d->mt2;
   o->JMNUM = d->jmnum; o->next_step = 1;
   o->H = (real)d->h; o->DT = (real)d->dt;
   /* mod_auxil.F90:19-20 and aux1.F90:36-46 */
-  o->A = (real)-1. / (real)24.;
-  o->B = (real)9. / (real)8.;
+  o->A = o->SECOND ? (real)0. : (real)-1. / (real)24.;
+  o->B = o->SECOND ? (real)1. : (real)9. / (real)8.;
   o->MX1 = o->MX - 1;
   o->PMLL = 1 - o->T; o->PMLXH = o->MX + o->T; o->PMLZH = o->MZ + o->T;
   o->NX = o->PMLXH - o->PMLL + 1; o->NZ = o->PMLZH + 1;

@@ -100,7 +100,7 @@ class OracleSim:
         self.job = job
         self.nz, self.nx = hostsetup.grid_shape(job)
         d = Desc(job.mx, job.mzp, job.tpml, int(job.stress_imaging), job.lin_x, job.lin_z,
-                 job.ktbo, job.mt2, job.table.shape[0], int(job.afda), job.h,
+                 job.ktbo, job.mt2, job.table.shape[0], int(job.variant), job.h,
                  float(np.float32(job.dt)) if precision == "f32" else job.dt)
         h = C.c_void_p()
         rc = self.lib.oracle_create(C.byref(d), C.byref(h))

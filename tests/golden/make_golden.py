@@ -38,7 +38,7 @@ REF = os.path.join(ROOT, "oracle", "_ref", "fd2d")
 REF_FMA = os.path.join(ROOT, "oracle", "_ref", "fd2d_fma")
 
 SCALARS = ("name", "mx", "mz", "h", "dt", "mt", "x_source", "z_source", "tpml", "stress_imaging",
-           "afda", "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo", "omg", "wb",
+           "variant", "r", "tau", "pow", "sf", "wc", "key_sour", "qr", "xbmin", "ktbo", "omg", "wb",
            "thpple", "thssle")
 
 

@@ -36,14 +36,14 @@ def tiny_edges(steps=150, tpml=5, mx=36, mz=24, f0=90.0, seed=7) -> Job:
     return models.make_job("EDGES", rocks, jm, h, steps, f0, xs, zs, recs, tpml=tpml, key_tlp=True)
 
 
-def tiny_layered_nosi(steps=120, afda=0, **kw) -> Job:
+def tiny_layered_nosi(steps=120, variant=0, **kw) -> Job:
     """Explicit free surface (no stress imaging, MZ' = MZ): the surface rows 0
     and 1 use their own one-sided stencils. Source 3 cells deep (rows 0 and 1
-    cannot hold it), receivers on rows 0 and 1. afda selects the reference's
-    AFDA_ZZ / AFDA_XZ stencil options."""
+    cannot hold it), receivers on rows 0 and 1. variant selects the reference's
+    AFDA_ZZ / AFDA_XZ / SECOND build options."""
     job = tiny_layered(steps=steps, **kw)
     job.stress_imaging = False
-    job.afda = afda
+    job.variant = variant
     job.z_source = 3.0 * job.h
     job.receivers = [(r[0], k % 2, r[2], k % 2) for k, r in enumerate(job.receivers)]
     return job
@@ -60,7 +60,15 @@ def tiny_edges_nosi(steps=150, **kw) -> Job:
 
 ALL = {"layered": tiny_layered, "homog": tiny_homog, "edges": tiny_edges,
        "layered_nosi": tiny_layered_nosi, "edges_nosi": tiny_edges_nosi,
-       "layered_nosi_afda": lambda: tiny_layered_nosi(afda=3)}
+       "layered_nosi_afda": lambda: tiny_layered_nosi(variant=3),
+       "layered_second": lambda: _second(tiny_layered()),
+       "layered_nosi_second": lambda: tiny_layered_nosi(variant=4)}
+
+
+def _second(job: Job) -> Job:
+    """The reference's second-order build (-DSECOND: A = 0, B = 1)."""
+    job.variant = 4
+    return job
 
 
 def medium_layered(steps=200, tpml=10, mx=200, mz=120, seed=5, f0=40.0) -> Job:

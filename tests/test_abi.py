@@ -46,7 +46,7 @@ def test_create_rejects_bad_descriptions():
     bad = [
         lib.Desc(4, 20, 8, 1, 2, 2, 0, 10, 1, 0, 1.0, 1e-4),     # MX too small
         lib.Desc(40, 30, 0, 1, 2, 2, 0, 10, 1, 0, 1.0, 1e-4),    # TPML 0
-        lib.Desc(40, 30, 8, 0, 2, 2, 0, 10, 1, 4, 1.0, 1e-4),    # unknown variant bit
+        lib.Desc(40, 30, 8, 0, 2, 2, 0, 10, 1, 8, 1.0, 1e-4),    # unknown variant bit
         lib.Desc(40, 30, 8, 1, 2, 29, 0, 10, 1, 0, 1.0, 1e-4),   # source row past MZ'-2
         lib.Desc(40, 30, 8, 1, 60, 5, 0, 10, 1, 0, 1.0, 1e-4),   # source column outside
     ]

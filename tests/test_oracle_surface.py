@@ -42,12 +42,12 @@ def dg(z, deg):
     return out
 
 
-def one_step(afda, **fields):
+def one_step(variant, **fields):
     """One oracle step (FP64) of a homogeneous non-imaging grid from the given
     initial fields (functions of depth in cells), with a silent source."""
     job = jobs.tiny_homog(steps=4)
     job.stress_imaging = False
-    job.afda = afda
+    job.variant = variant
     job.z_source = 6.0 * job.h
     job.stf_v = np.zeros_like(job.stf_v)
     job.receivers = []
@@ -107,27 +107,27 @@ def test_row0_qx_u_interior_cubic_and_strips_quadratic():
             np.testing.assert_allclose(u[0, c] / tab[MO_INDEX["REL1_U"]], dg(0.5, deg), rtol=1e-12)
 
 
-@pytest.mark.parametrize("afda,deg,lo", [(0, 2, None), (1, 4, 2)])
-def test_row0_strain_zz(afda, deg, lo):
+@pytest.mark.parametrize("variant,deg,lo", [(0, 2, None), (1, 4, 2)])
+def test_row0_strain_zz(variant, deg, lo):
     """STRAIN_UW_ZZ_0: the 2-point difference (exact for quadratics) or, with
     AFDA_ZZ, the 5-point one-sided form (exact for quartics), at depth 1/2."""
-    job, sim, tab = one_step(afda, W=lambda l: g(l, deg) + 0.25)
+    job, sim, tab = one_step(variant, W=lambda l: g(l, deg) + 0.25)
     lo = 2 - job.tpml if lo is None else lo
-    hi = job.mx - 1 + (job.tpml if not afda else 0)
+    hi = job.mx - 1 + (job.tpml if not variant else 0)
     c = cols(job, lo, hi)
     ezz = sim.field("TZZ")[0, c] / (np.float64(job.dt) * tab[MO_INDEX["SIG_ZZ_2"]])
     np.testing.assert_allclose(ezz, dg(0.5, deg), rtol=1e-12)
 
 
-@pytest.mark.parametrize("afda,prof,want", [
+@pytest.mark.parametrize("variant,prof,want", [
     (0, lambda z: g(z, 2), dg(1.0, 2)),
     (2, lambda z: B * z ** 2 + C3 * z ** 3, 2 * B + 3 * C3),   # U'(0) = 0 = -W_x(0)
 ])
-def test_row1_strain_xz(afda, prof, want):
+def test_row1_strain_xz(variant, prof, want):
     """STRAIN_UW_XZ_1 at depth 1 from U at depths 1/2, 3/2, ...: the 2-point
     difference, or with AFDA_XZ the 4-point form corrected by the surface
     condition U_z(0) = -W_x(0) (exact for cubics that satisfy it)."""
-    job, sim, tab = one_step(afda, U=lambda l: prof(l + 0.5))
+    job, sim, tab = one_step(variant, U=lambda l: prof(l + 0.5))
     c = cols(job, 3, job.mx - 1)
     exz = sim.field("TXZ")[1, c] / (np.float64(job.dt) * tab[MO_INDEX["MU"]])
     np.testing.assert_allclose(exz, want, rtol=1e-12)
