@@ -22,8 +22,8 @@ EXPORTS = (
     "bq_create", "bq_destroy", "bq_upload_model", "bq_build_pml", "bq_upload_pml", "bq_get_pml",
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
-    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells", "bq_probe_copy",
-    "bq_probe_march", "bq_build_bottom_nr", "bq_nonref",
+    "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells",
+    "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
 )
 
 
@@ -72,12 +72,13 @@ def load(path: str = LIBPATH):
         "bq_grid_shape": ([p, p, p], C.c_int),
         "bq_enable_timing": ([p, C.c_int], C.c_int),
         "bq_kernel_times": ([p, p], C.c_int),
-        "bq_probe_copy": ([C.c_longlong, C.c_int, p], C.c_int),
         "bq_kernel_cells": ([p, p], C.c_int),
-        "bq_probe_march": ([C.c_int, C.c_int, C.c_int, p], C.c_int),
         "bq_build_bottom_nr": ([p, p, p, p], C.c_int),
         "bq_nonref": ([C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, p, p],
                       C.c_int),
+        "bq_write_seismograms": ([C.c_char_p, p, C.c_int, C.c_int, C.c_int, C.c_float, p, C.c_int,
+                                  C.c_int, p], C.c_int),
+        "bq_format_e13_6": ([C.c_float, C.c_char_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -100,25 +101,6 @@ def nonref(key: int, h, dt, vel, vels, velp, prm) -> np.ndarray:
     if rc:
         raise BiotError(f"bq_nonref failed ({rc})")
     return out
-
-
-def probe_march(nx: int, nz: int, iters: int = 20) -> float:
-    """Read+write GB/s of the fused tiles' row-streaming access pattern (nine
-    arrays read, eight written, no arithmetic) on an nx x nz grid."""
-    out = np.zeros(1, dtype=np.float64)
-    rc = load().bq_probe_march(int(nx), int(nz), int(iters), out.ctypes.data_as(C.c_void_p))
-    if rc:
-        raise BiotError(f"bq_probe_march failed ({rc})")
-    return float(out[0])
-
-
-def probe_copy(nfloats: int, iters: int = 20) -> float:
-    """Read+write GB/s of a plain device-to-device copy of nfloats floats."""
-    out = np.zeros(1, dtype=np.float64)
-    rc = load().bq_probe_copy(int(nfloats), int(iters), out.ctypes.data_as(C.c_void_p))
-    if rc:
-        raise BiotError(f"bq_probe_copy failed ({rc})")
-    return float(out[0])
 
 
 def _ptr(a: np.ndarray):

@@ -1,4 +1,5 @@
-"""Build the HIP library ``biotfd/libbiotfd.so`` for gfx950 (and the oracle).
+"""Build the HIP library ``biotfd/libbiotfd.so`` for gfx950, and beside it the
+diagnostics library ``biotfd/libbiotfd_probes.so`` (bandwidth probes).
 
 Flags that matter for correctness:
   -ffp-contract=off   every a*b+c rounds twice, as the reference's source order
@@ -17,24 +18,31 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BIOTFD_ARCH", "gfx950")
 OUT = os.path.join(HERE, "biotfd", "libbiotfd.so")
-SOURCES = [os.path.join(HERE, "csrc", "bq_device.hip"), os.path.join(HERE, "csrc", "bq_host.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("bq_device.hip", "bq_host.cpp", "bq_output.cpp")]
+PROBES_OUT = os.path.join(HERE, "biotfd", "libbiotfd_probes.so")
+PROBES_SOURCES = [os.path.join(HERE, "csrc", "bq_probes.hip")]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
-    "-I", os.path.join(ROOT, "include"),
+    "-I", os.path.join(ROOT, "include"), "-pthread",
 ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", tmp]
+def _link(sources, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, *sources, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    _link(SOURCES, OUT, verbose)
+    _link(PROBES_SOURCES, PROBES_OUT, verbose)
     return OUT
 
 

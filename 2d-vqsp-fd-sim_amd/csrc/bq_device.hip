@@ -997,53 +997,6 @@ __global__ void k_sample(KParams k, int itile) {
 
 #undef FA
 
-// Plain streaming copy, used to calibrate HBM traffic counters and to measure
-// the achievable copy bandwidth of the device.
-__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a,
-                                                   float4 *__restrict__ b, size_t n4) {
-  // each block moves 4 x 256 float4 (16 KiB), four independent loads in flight per lane
-  const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
-  float4 v[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const size_t t = base + 256 * u;
-    v[u] = t < n4 ? a[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const size_t t = base + 256 * u;
-    if (t < n4) b[t] = v[u];
-  }
-}
-
-// Row-streaming probe: blocks of kFW columns x kFR rows march down nine input
-// arrays and write eight output arrays two rows behind, with no stencil work -
-// the access pattern of the fused tiles without their arithmetic. Columns
-// [0, nbx*kFW), rows [4, 4 + nbz*kFR + 4) of [nz][pitch] arrays.
-struct MarchArrays {
-  const float *in[9];
-  float *out[8];
-};
-
-__global__ __launch_bounds__(kFW) void k_march_probe(MarchArrays A, int pitch, int nbx) {
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = tile % nbx, bz = tile / nbx;
-  const int c = bx * kFW + threadIdx.x;
-  const int a = 4 + bz * kFR;
-  for (int s = a; s < a + kFR + 4; ++s) {
-    const size_t r = (size_t)s * pitch + c;
-    float v = 0.0f;
-#pragma unroll
-    for (int f = 0; f < 9; ++f) v += A.in[f][r];
-    const int l = s - 2;
-    if (l >= a && l < a + kFR) {
-      const size_t rl = (size_t)l * pitch + c;
-#pragma unroll
-      for (int f = 0; f < 8; ++f) A.out[f][rl] = v;
-    }
-  }
-}
-
 }  // namespace
 
 // ===================================================================== host
@@ -1098,6 +1051,16 @@ int dalloc(bq_handle *h, Tp **p, size_t n) {
   h->allocs.push_back(q);
   *p = (Tp *)q;
   return 0;
+}
+
+// Frees one buffer of the handle (after the stream has drained).
+void dfree(bq_handle *h, void *p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it == h->allocs.end()) return;
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(p);
+  h->allocs.erase(it);
 }
 
 }  // namespace
@@ -1436,8 +1399,18 @@ int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *
                      const int32_t *irecz, const int32_t *lrecz) {
   if (!h || mr < 0 || (mr > 0 && (!irecx || !lrecx || !irecz || !lrecz)))
     return fail(h, -1, "bad receiver arguments");
+  // the trace buffer holds every step from the first: receivers are part of
+  // the set-up, as in the reference (read_open.F90:234-236)
+  if (h->next_step > 1) return fail(h, -6, "bq_set_receivers after the first step");
   std::vector<int4> r(mr > 0 ? mr : 1);
   for (int j = 0; j < mr; ++j) r[j] = make_int4(irecx[j], lrecx[j], irecz[j], lrecz[j]);
+  // a second call before the first step replaces the previous buffers
+  dfree(h, (void *)h->kp.rec);
+  dfree(h, h->kp.traces);
+  h->kp.rec = nullptr;
+  h->kp.traces = nullptr;
+  h->kp.MR = 0;
+  h->mr = 0;
   int4 *rd = nullptr;
   float *tr = nullptr;
   if (dalloc(h, &rd, (size_t)(mr > 0 ? mr : 1))) return -101;
@@ -1539,71 +1512,6 @@ int bq_kernel_times(bq_handle *h, double out[3]) {
   if (!h->timed_steps) return fail(h, -6, "no timed run");
   for (int q = 0; q < 3; ++q) out[q] = h->ktime[q];
   return 0;
-}
-
-int bq_probe_copy(long long nfloats, int iters, double *gbps) {
-  if (nfloats < 4 || iters < 1 || !gbps) return -1;
-  const size_t n4 = (size_t)nfloats / 4;
-  float4 *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, n4 * 16) != hipSuccess) return -101;
-  if (hipMalloc(&b, n4 * 16) != hipSuccess) { (void)hipFree(a); return -101; }
-  (void)hipMemset(a, 0, n4 * 16);
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  const int blocks = (int)((n4 + 1023) / 1024);
-  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);
-  (void)hipEventRecord(e0, 0);
-  for (int r = 0; r < iters; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, a, b, n4);
-  (void)hipEventRecord(e1, 0);
-  (void)hipEventSynchronize(e1);
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  *gbps = 2.0 * n4 * 16 * iters / (ms * 1e-3) / 1e9;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
-  return 0;
-}
-
-int bq_probe_march(int nx, int nz, int iters, double *gbps) {
-  if (nx < kFW || nz < kFR + 12 || iters < 1 || !gbps) return -1;
-  const int pitch = (nx + 63) / 64 * 64;
-  const int nbx = nx / kFW, nbz = (nz - 8) / (kFR + 4);
-  // rows touched: 4 .. 4 + nbz*kFR + 3 < nz; columns < nbx*kFW <= nx <= pitch
-  const size_t n = (size_t)pitch * nz;
-  std::vector<float *> buf(17, nullptr);
-  int rc = 0;
-  for (auto &p : buf) {
-    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { p = nullptr; rc = -101; break; }
-    (void)hipMemset(p, 0, n * sizeof(float));
-  }
-  if (rc == 0) {
-    MarchArrays A{};
-    for (int f = 0; f < 9; ++f) A.in[f] = buf[f];
-    for (int f = 0; f < 8; ++f) A.out[f] = buf[9 + f];
-    const int blocks = nbx * nbz;
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx);
-    (void)hipEventRecord(e0, 0);
-    for (int r = 0; r < iters; ++r)
-      hipLaunchKernelGGL(k_march_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx);
-    (void)hipEventRecord(e1, 0);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    const double cells = (double)nbx * kFW * nbz * kFR;
-    *gbps = cells * 17.0 * sizeof(float) * iters / (ms * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (hipGetLastError() != hipSuccess) rc = -100;
-  }
-  for (auto p : buf)
-    if (p) (void)hipFree(p);
-  return rc;
 }
 
 int bq_sync(bq_handle *h) {

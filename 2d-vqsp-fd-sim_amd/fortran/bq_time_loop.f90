@@ -4,11 +4,14 @@
 ! Drop-in for SUBROUTINE TIME_LOOP (time_loop.F90:1-209): same name, no
 ! arguments, same module state. The set-up the reference driver already did
 ! (READ_MODEL, READ_MODEL_PAR, OPEN_SR_FILE, PRE_NR_COEF1, PPML) is handed to
-! the device once; the steps run there; the reference's own writers
-! STORE_SEISMOGRAMS, STORE_SEISMOGRAMS_PRESSURE and STORE_SNAPSHOTS produce the
-! output files unchanged. Receiver samples come back once per chunk of steps
-! and are placed at the receiver nodes of the host arrays before each writer
-! call, so the writers see exactly the values they would have read.
+! the device once; the steps run there. Receiver samples come back once per
+! chunk of steps. STORE_CHUNK below appends the chunk's lines to the receiver
+! files with one OPEN per file per chunk: the same values and the same format
+! STORE_SEISMOGRAMS and STORE_SEISMOGRAMS_PRESSURE write with one OPEN per file
+! per step (store_seismograms.F90:54-153, store_seismograms_pressure.F90:45-81).
+! Built with -DBQ_REFERENCE_WRITERS, the reference's own writers run instead,
+! step by step, after the samples are placed at the receiver nodes of the host
+! arrays. STORE_SNAPSHOTS writes the snapshots unchanged.
 ! The STOP file (time_loop.F90:79-86) is polled at the start of each chunk.
 !=======================================================================
 SUBROUTINE  TIME_LOOP
@@ -32,7 +35,7 @@ SUBROUTINE  TIME_LOOP
                           ODD2_XL, ODD2_XH, ODA2_ZH, ODB2_ZH, ODC2_ZH, ODD2_ZH
   USE NONREF_BOUND, ONLY: ABW01, ABW02, ABW10, ABW11, ABW12, ABW20, ABW21, ABW22,         &
                           ABH01, ABH02, ABH10, ABH11, ABH12, ABH20, ABH21, ABH22
-  USE WAVEFIELD   , ONLY: UM, WM, QX, QZ
+  USE WAVEFIELD   , ONLY: UM, WM, QX, QZ, SSU, SSW, SSQX, SSQZ
   USE STRESSFIELD , ONLY: TXX, PRES
   USE INTERFACES  , ONLY: STORE_SEISMOGRAMS, STORE_SEISMOGRAMS_PRESSURE, STORE_SNAPSHOTS
 
@@ -53,6 +56,7 @@ SUBROUTINE  TIME_LOOP
   REAL(C_FLOAT) :: ABW(8), ABH(8)
   INTEGER       :: RC, I0, I1, J, S, K, OFF, N
   LOGICAL       :: EX, SEIS
+  CHARACTER(LEN=20), EXTERNAL :: STR
 
   CALL TL_ALLOC
 
@@ -120,7 +124,7 @@ SUBROUTINE  TIME_LOOP
   RC = BQ_SET_SOURCE(HB, SOURTF_A)
   IF (RC /= 0) CALL FAIL('BQ_SET_SOURCE', RC)
 
-  SEIS = KEY_TLV .AND. MR > 0
+  SEIS = (KEY_TLV .OR. KEY_TLP) .AND. MR > 0
   IF (SEIS) THEN
     RC = BQ_SET_RECEIVERS(HB, MR, IRECX, LRECX, IRECZ, LRECZ)
     IF (RC /= 0) CALL FAIL('BQ_SET_RECEIVERS', RC)
@@ -148,6 +152,12 @@ SUBROUTINE  TIME_LOOP
       RC = BQ_SYNC(HB)
       IF (RC /= 0) CALL FAIL('BQ_SYNC', RC)
     END IF
+#if(!BQ_REFERENCE_WRITERS)
+    DO ITILE = I0, I1
+      WRITE (11,'(I6)') ITILE
+    END DO
+    IF (SEIS) CALL STORE_CHUNK(I0, I1)
+#else
     DO ITILE = I0, I1
       WRITE (11,'(I6)') ITILE
       IF (SEIS) THEN
@@ -166,10 +176,11 @@ SUBROUTINE  TIME_LOOP
             TXX (IRECZ(J), LRECX(J)) = -TR(6, J, S)
           END IF
         END DO
-        CALL STORE_SEISMOGRAMS(ITILE)
+        IF (KEY_TLV) CALL STORE_SEISMOGRAMS(ITILE)
         IF (KEY_TLP) CALL STORE_SEISMOGRAMS_PRESSURE(ITILE)
       END IF
     END DO
+#endif
     IF (KEY_SNV .AND. MOD(I1, IPAS2) == 0) THEN
       RC = BQ_GET_FIELD(HB, BQ_U, UM)
       IF (RC == 0) RC = BQ_GET_FIELD(HB, BQ_W, WM)
@@ -192,6 +203,64 @@ CONTAINS
     COEF(K) = C_LOC(PBUF(OFF))
     OFF = OFF + SIZE(A)
   END SUBROUTINE PUT
+
+  ! Receiver files of steps I0..I1 from the chunk's samples TR(:,J,S): per
+  ! step the lines STORE_SEISMOGRAMS (store_seismograms.F90:54-100, 140-153)
+  ! and STORE_SEISMOGRAMS_PRESSURE (store_seismograms_pressure.F90:45-81)
+  ! append, with the derivatives formed from the previous samples SSU, SSW,
+  ! SSQX, SSQZ as there; each file is opened once for the whole chunk.
+  SUBROUTINE STORE_CHUNK(I0, I1)
+    INTEGER, INTENT(IN) :: I0, I1
+    INTEGER  :: J, T, S
+    REAL(PP) :: DU, DW, DQX, DQZ, P, X
+    LOGICAL  :: TLQ
+    TLQ = KEY_TLV .AND. KEY_TLQ
+    DO J = 1, MR
+      IF (KEY_TLV) OPEN (21, FILE='REC_UW_'//TRIM(STR(J))//'.asc', STATUS='UNKNOWN',  &
+                         POSITION='APPEND')
+      IF (TLQ) OPEN (22, FILE='REC_Q_'//TRIM(STR(J))//'.asc', STATUS='UNKNOWN',       &
+                     POSITION='APPEND')
+      IF (KEY_TLP) OPEN (27, FILE='REC_STRESS_'//TRIM(STR(J))//'.asc', STATUS='UNKNOWN', &
+                         POSITION='APPEND')
+      DO T = I0, I1
+        S = T - I0 + 1
+        DU = 0.
+        DW = 0.
+        DQX = 0.
+        DQZ = 0.
+        IF (KEY_TLV .AND. INSIDE(IRECX(J), LRECX(J))) THEN
+          DU = (TR(1, J, S) - SSU(J))/DT
+          SSU(J) = TR(1, J, S)
+          IF (TLQ) THEN
+            DQX = (TR(3, J, S) - SSQX(J))/DT
+            SSQX(J) = TR(3, J, S)
+          END IF
+        END IF
+        IF (KEY_TLV .AND. INSIDE(IRECZ(J), LRECZ(J))) THEN
+          DW = (TR(2, J, S) - SSW(J))/DT
+          SSW(J) = TR(2, J, S)
+          IF (TLQ) THEN
+            DQZ = (TR(4, J, S) - SSQZ(J))/DT
+            SSQZ(J) = TR(4, J, S)
+          END IF
+        END IF
+        IF (KEY_TLV) WRITE (21, '(5(1X,E13.6))') DT*(T), SSU(J), SSW(J), DU, DW
+        IF (TLQ) WRITE (22, '(5(1X,E13.6))') DT*(T), SSQX(J), SSQZ(J), DQX, DQZ
+        IF (KEY_TLP) THEN
+          P = 0.
+          X = 0.
+          IF (INSIDE(IRECZ(J), LRECX(J))) THEN
+            P = TR(5, J, S)
+            X = TR(6, J, S)
+          END IF
+          WRITE (27, '(3(1X,E13.6))') DT*(T), P, X
+        END IF
+      END DO
+      IF (KEY_TLV) CLOSE (21)
+      IF (TLQ) CLOSE (22)
+      IF (KEY_TLP) CLOSE (27)
+    END DO
+  END SUBROUTINE STORE_CHUNK
 
   LOGICAL FUNCTION INSIDE(I, L)
     INTEGER, INTENT(IN) :: I, L

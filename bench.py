@@ -31,9 +31,15 @@ sys.path.insert(0, os.path.join(ROOT, "2d-vqsp-fd-sim_amd"))
 from biotfd import formats, hostsetup, lib, models  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-B_STEP = 104          # algorithmic bytes per cell per step, two half-steps (indexed materials)
-B_FUSED = 68          # fused cell: 8 fields read, 8 fields written, 1 int32 index
-B_HALF = 52           # one half-step: 4 fields read, 4 read and written, 1 int32 index
+# Algorithmic bytes of one cell (indexed materials): each half-step reads the
+# other group's 4 fields, reads and writes its own 4 and reads the int32
+# material index, 52 B; a whole step (both half-steps) is 104 B.
+B_STEP = 104
+B_HALF = 52
+# What the fused interior kernel must move at least for one cell-step: 8 fields
+# read once, 8 written once, 1 index (the other group's fields are consumed in
+# registers and LDS within the same sweep).
+B_FUSED_MIN = 68
 TIMING_STRIDE = 8     # kernel durations from HIP events around every 8th step
 
 
@@ -124,9 +130,13 @@ def main():
     u = sim.field("U")
     if not np.isfinite(u).all():
         raise SystemExit("non-finite field after run")
+    # k_step advances the fused cells by a whole step (104 B each) and the
+    # other cells by the stress half-step (52 B each)
     dom_name = "k_step"
-    dom_bytes = B_FUSED * kcells["k_fused"] + B_HALF * kcells["k_stress"]
-    achieved = dom_bytes / (ktimes[dom_name] * 1e-3) / 1e9
+    dom_bytes = B_STEP * kcells["k_fused"] + B_HALF * kcells["k_stress"]
+    dom_s = ktimes[dom_name] * 1e-3
+    achieved = dom_bytes / dom_s / 1e9
+    min_bytes = B_FUSED_MIN * kcells["k_fused"] + B_HALF * kcells["k_stress"]
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tr_path):
@@ -135,8 +145,9 @@ def main():
 
     copy_gbs = march_gbs = None
     if args.copy_probe:
-        copy_gbs = lib.probe_copy(1 << 28, 20)
-        march_gbs = lib.probe_march(nx, nz, 20)
+        from biotfd import probes
+        copy_gbs = probes.probe_copy(1 << 28, 20)
+        march_gbs, _ = probes.probe_march(nx, nz, 20)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -159,13 +170,20 @@ def main():
                                f"TPML=20, free surface (stress imaging)",
                    "grid_cells": cells, "nxc": nx, "nzc": nz, "jmnum": int(job.table.shape[0]),
                    "b_alg_bytes_per_cell_step": B_STEP, "receivers": len(job.receivers)},
+        # whole step, all launches: 104 B x NXc x NZc per step over ms_per_step
         "hbm_roofline_frac_step": round(gcell * 1e9 * B_STEP / (HBM_PEAK_GBS * 1e9), 4),
         "kernel_ms": {n: round(t, 5) for n, t in ktimes.items()},
         "kernel_cells": kcells,
-        "roofline": {"bound": "hbm", "kernel": dom_name, "algorithmic_bytes": int(dom_bytes),
+        "roofline": {"bound": "hbm", "kernel": dom_name,
+                     "basis": "104 B per fused cell-step + 52 B per other stress cell, "
+                              "per k_step launch",
+                     "algorithmic_bytes": int(dom_bytes),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
+        # the least k_step can move (68 B per fused cell-step): its rate
+        "k_step_min_bytes": {"bytes": int(min_bytes), "gbs": round(min_bytes / dom_s / 1e9, 1),
+                             "frac": round(min_bytes / dom_s / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     if copy_gbs is not None:

@@ -113,7 +113,9 @@ int bq_set_source(bq_handle *h, const float *sourtf);
 
 /* Receivers (read_open.F90:234-236): 1-based x index and z index of the U
  * node (IRECX, LRECX) and of the W node (IRECZ, LRECZ). Allocates a device
- * trace buffer for MT2 steps. */
+ * trace buffer for MT2 steps. Part of the set-up: rejected (-6) once a step
+ * has run; a second call before the first step replaces the receivers and
+ * frees the previous buffers. */
 int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *lrecx,
                      const int32_t *irecz, const int32_t *lrecz);
 
@@ -145,6 +147,26 @@ int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float 
 int bq_nonref(int key, float h, float dt, float vel, float vels, float velp, const float prm[4],
               float out[8]);
 
+/* Output writers (host side, no handle). STORE_SEISMOGRAMS and
+ * STORE_SEISMOGRAMS_PRESSURE (store_seismograms.F90:47-154,
+ * store_seismograms_pressure.F90:45-83) for a chunk of steps: samples is
+ * [nsteps][mr][6] as bq_get_traces returns it for steps first_step ..
+ * first_step+nsteps-1; appends one line per step to REC_UW_<j>.asc (t, U, -W
+ * and their time derivatives), REC_Q_<j>.asc with key_tlq (QX, -QZ) and
+ * REC_STRESS_<j>.asc with key_tlp (PRES, -TXX) in directory dir, in the
+ * reference's "(n(1X,E13.6))" format, opening each file once per call.
+ * inside: [mr][2] int32, whether the U node and the W node lie in the grid
+ * (NULL: all); outside nodes give 0 samples and derivatives. state: [mr][4]
+ * binary32 SSU, SSW, SSQX, SSQZ (the previous samples the derivatives are
+ * formed from), zero before the first step, updated in place. Receivers are
+ * written by up to 16 host threads. */
+int bq_write_seismograms(const char *dir, const float *samples, int first_step, int nsteps,
+                         int mr, float dt, const int32_t *inside, int key_tlq, int key_tlp,
+                         float *state);
+/* The Fortran E13.6 edit of one binary32 value as the reference's runtime
+ * writes it (13 characters and a terminating NUL into out[14]). */
+int bq_format_e13_6(float x, char *out);
+
 /* Informational. */
 const char *bq_last_error(bq_handle *h);
 int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
@@ -163,19 +185,6 @@ int bq_kernel_times(bq_handle *h, double out[3]);
  * k_velocity. */
 int bq_kernel_cells(bq_handle *h, long long out[3]);
 void bq_destroy(bq_handle *h);
-
-/* Diagnostic with no counterpart in the reference: streams nfloats binary32
- * values from one device buffer to another iters times (kernel k_copy_probe)
- * and reports read+write GB/s. Used to measure the achievable HBM bandwidth
- * and to calibrate hardware traffic counters. */
-int bq_probe_copy(long long nfloats, int iters, double *gbps);
-
-/* Diagnostic with no counterpart in the reference: the access pattern of the
- * fused step tiles without their arithmetic (kernel k_march_probe). Blocks of
- * 512 columns x 128 rows march down nine [nz][nx] arrays and write eight, one
- * row at a time; reports read+write GB/s, the practical bandwidth ceiling of
- * that row-streaming pattern. Needs nx >= 512, nz >= 140. */
-int bq_probe_march(int nx, int nz, int iters, double *gbps);
 
 #ifdef __cplusplus
 }

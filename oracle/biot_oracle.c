@@ -15,6 +15,9 @@
  * Compiled twice: REAL=float (the reference's PP=SP build, which it must match
  * bit for bit) and REAL=double (the FP64 oracle). Both builds must be compiled
  * without floating-point contraction so every operation rounds as written.
+ * The interior rows of each half-step are independent of one another and are
+ * shared out over OpenMP threads (one_step); each cell's expression is the
+ * same, so the results do not depend on the thread count.
  *
  * Grid convention (aux1.F90:36-46, domain_decomposition.F90:146-154): x index
  * i in [PMLL, PMLXH] = [1-T, MX+T], z index l in [0, PMLZH]; every full-grid
@@ -22,7 +25,9 @@
  *
  * Parity status: pinned bit-exactly against snapshots of the reference built
  * from its own sources (oracle/Makefile -> oracle/_ref/fd2d), see
- * tests/golden/make_golden.py and tests/test_oracle_golden.py. The surface
+ * tests/golden/make_golden.py and tests/test_oracle_golden.py, including the
+ * CPU-runnable workload at its full size (400 x 400 cells, 1500 steps) by the
+ * digests of every file the reference writes. The surface
  * rows without stress imaging (HPLANE_0/1_* non-imaging branches, with the
  * AFDA_ZZ / AFDA_XZ options as run-time bits) follow the subroutine bodies;
  * the reference itself faults in that variant (its INH_QZ_2ND_0 interface
@@ -78,6 +83,10 @@ typedef struct {
   real *f[F_NFIELDS];
   /* row temporaries of MODULE STRAINFIELD (mod_strainfield.F90:7-9) */
   real *EXX, *EZZ, *EXZ, *QXX, *QZZ;
+  /* per-thread copies of the row temporaries for the interior rows, which
+   * the OpenMP loops of one_step share out: 5 x NX reals per thread */
+  real *rowpool;
+  int nrowpool;
   /* PML coefficients (mod_pml.F90:7-19): XL/XH are (0:T[+1], NZ), ZH (0:T[+1], NX) */
   real *ODA_XL, *ODB_XL, *ODC_XL, *ODD_XL, *ODA_XH, *ODB_XH, *ODC_XH, *ODD_XH;
   real *ODA_ZH, *ODB_ZH, *ODC_ZH, *ODD_ZH;
@@ -1036,15 +1045,37 @@ static void sample_receivers(orc *o, int itile) {
 }
 
 /* One DO ITILE iteration, time_loop.F90:91-190 (serial build). */
+/* Row temporaries of OpenMP thread `tid` (the pool grows to the team size). */
+static rowtmp thread_rows(orc *o, int tid) {
+  real *b = o->rowpool + (size_t)tid * 5 * o->NX;
+  rowtmp t = {b, b + o->NX, b + 2 * (size_t)o->NX, b + 3 * (size_t)o->NX, b + 4 * (size_t)o->NX};
+  return t;
+}
+
+/* One DO ITILE iteration (time_loop.F90:91-190). The interior rows of each
+ * half-step (HPLANE_IC_*, rows 2..MZ'-2) read only the other half-step's
+ * fields and update only their own row and their own rows of the PML
+ * memories, so they are independent: they are shared out over OpenMP threads,
+ * each with its own row temporaries. Every cell is still evaluated by the same
+ * expression, so the result does not depend on the thread count. */
 static void one_step(orc *o, int itile) {
   const int MZ = o->MZ, PMLZH1 = o->PMLZH - 1;
   rowtmp t = {o->EXX, o->EZZ, o->EXZ, o->QXX, o->QZZ};
   real S = o->SOURTF[(size_t)(itile - 1) * 2 + 0];
   real F = o->SOURTF[(size_t)(itile - 1) * 2 + 1];
-  /* stress half-step */
-  for (int L = 2; L <= o->LIN_Z - 1; ++L) hplane_ic_stress(o, L, &t, 0, 0, 0);
-  hplane_ic_stress(o, o->LIN_Z, &t, 1, S, F);
-  for (int L = o->LIN_Z + 1; L <= MZ - 2; ++L) hplane_ic_stress(o, L, &t, 0, 0, 0);
+  const int nth = omp_get_max_threads();
+  if (nth > o->nrowpool) {
+    free(o->rowpool);
+    o->rowpool = (real *)zalloc(sizeof(real) * 5 * (size_t)o->NX * (size_t)nth);
+    o->nrowpool = nth;
+  }
+  /* stress half-step; the source row is HPLANE_LIN_STRESS */
+#pragma omp parallel
+  {
+    rowtmp tt = thread_rows(o, omp_get_thread_num());
+#pragma omp for schedule(static)
+    for (int L = 2; L <= MZ - 2; ++L) hplane_ic_stress(o, L, &tt, L == o->LIN_Z, S, F);
+  }
   if (o->SI) {
     hplane_01_stress_si(o);
   } else {
@@ -1057,6 +1088,7 @@ static void one_step(orc *o, int itile) {
   /* bottom store */
   store_bottom(o);
   /* velocity half-step */
+#pragma omp parallel for schedule(static)
   for (int L = 2; L <= MZ - 2; ++L) hplane_ic_velocity(o, L);
   if (o->SI) {
     hplane_1_velocity_si(o);
@@ -1253,6 +1285,7 @@ EXPORT void oracle_destroy(orc *o) {
   free(o->JM);
   for (int k = 0; k < T_N; ++k) free(o->tab[k]);
   for (int k = 0; k < F_NFIELDS; ++k) free(o->f[k]);
+  free(o->rowpool);
   real *arrs[] = {o->EXX, o->EZZ, o->EXZ, o->QXX, o->QZZ, o->ODA_XL, o->ODB_XL, o->ODC_XL,
                   o->ODD_XL, o->ODA_XH, o->ODB_XH, o->ODC_XH, o->ODD_XH, o->ODA_ZH, o->ODB_ZH,
                   o->ODC_ZH, o->ODD_ZH, o->ODA2_XL, o->ODB2_XL, o->ODC2_XL, o->ODD2_XL,

@@ -15,10 +15,18 @@ reference's own input files, runs both binaries, and stores
 
 Run in the container that holds the reference: python tests/golden/make_golden.py
 [case ...] (all cases without arguments).
+
+``python tests/golden/make_golden.py cfg0`` writes ``cfg0_digests.npz``: the
+CPU-runnable workload at its full size (homogeneous Biot full space, 400 x 400
+cells, dx = 1 m, 20-cell PML, 1500 steps, 100 receivers). Its outputs are too
+large to store, so the fixture holds the job and the SHA-256 digests of every
+file the reference writes: the snapshots SNAP.V / SNAP.Q at steps 1, 500, 1000
+and 1500 and the 300 receiver files.
 """
 from __future__ import annotations
 
 import glob
+import hashlib
 import json
 import os
 import subprocess
@@ -102,6 +110,11 @@ def run(exe, job, d, snap_steps):
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     only = set(sys.argv[1:])
+    if "cfg0" in only:
+        make_cfg0()
+        only.discard("cfg0")
+        if not only:
+            return
     for name, job in cases().items():
         if only and name not in only:
             continue
@@ -125,6 +138,58 @@ def main():
         print(f"{name}: {steps} steps, snapshots {snap_steps}, {os.path.getsize(path)} bytes")
 
 
+CFG0_SNAPS = (1, 500, 1000, 1500)
+
+
+def cfg0_job():
+    """The CPU-runnable workload at full size (models.homogeneous_job), with
+    every output switched on."""
+    from biotfd import models
+    job = models.homogeneous_job()
+    job.key_tlv = job.key_tlq = job.key_tlp = job.key_snv = True
+    job.ipas2 = 1
+    return job
+
+
+def sha256_file(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def make_cfg0():
+    job = cfg0_job()
+    with tempfile.TemporaryDirectory() as d:
+        # IPAS2 = 1 would write 1500 snapshot pairs; 500 gives 500, 1000, 1500
+        # and step 1 comes from a one-step run of the same job
+        job.ipas2 = 500
+        formats.write_job(job, d)
+        subprocess.run([REF], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=1200)
+        files = sorted(f for f in os.listdir(d) if f.startswith(("REC_", "SNAP.")))
+        digests = {f: sha256_file(os.path.join(d, f)) for f in files}
+    with tempfile.TemporaryDirectory() as d:
+        one = cfg0_job()
+        one.mt = float(np.float32(0.5 * np.float32(one.dt)))
+        assert one.mt2 == 1
+        one.key_tlv = one.key_tlp = False   # KEY_TLQ stays: it switches SNAP.Q on
+        formats.write_job(one, d)
+        subprocess.run([REF], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=600)
+        for f in ("SNAP.V000001", "SNAP.Q000001"):
+            digests[f] = sha256_file(os.path.join(d, f))
+    job.ipas2 = 1
+    meta = {k: getattr(job, k) for k in SCALARS}
+    meta["receivers"] = [list(map(int, r)) for r in job.receivers]
+    names = sorted(digests)
+    path = os.path.join(HERE, "cfg0_digests.npz")
+    np.savez_compressed(
+        path, meta=json.dumps(meta), jm=np.asarray(job.jm, np.int32),
+        table=np.asarray(job.table, np.float64), stf_t=np.asarray(job.stf_t, np.float32),
+        stf_v=np.asarray(job.stf_v, np.float32), snap_steps=np.asarray(CFG0_SNAPS, np.int32),
+        files=np.asarray(names), sha256=np.asarray([digests[n] for n in names]))
+    print(f"cfg0: {job.mt2} steps, {len(names)} digests, {os.path.getsize(path)} bytes")
+
+
 def load(name):
     """Job and arrays of one fixture."""
     z = np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False)
@@ -136,7 +201,9 @@ def load(name):
 
 
 def names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+    """The small snapshot fixtures (cfg0_digests.npz holds digests instead)."""
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz"))
+                  if not os.path.basename(p).startswith("cfg0"))
 
 
 if __name__ == "__main__":

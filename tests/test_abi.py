@@ -75,3 +75,17 @@ def test_nonref_matches_oracle(key):
             b = oracle.nonref(key, h, dt, vel, vs, vp, prm, "f32")
             np.testing.assert_array_equal(a, b)
             assert np.isfinite(a).all()
+
+
+def test_probes_library_is_separate():
+    """The bandwidth probes live in their own library, not in the time-loop ABI."""
+    from biotfd import probes
+    src = open(os.path.join(ROOT, "include", "biotfd_probes.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(bq_[a-z_0-9]+)\s*\(", src)))
+    assert names == sorted(probes.EXPORTS)
+    so = probes.load()
+    for n in names:
+        assert hasattr(so, n), n
+    assert not set(names) & set(declared_functions())
+    assert not any(hasattr(lib.load(), n) for n in names)

@@ -13,21 +13,26 @@ from biotfd import formats, hostsetup
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref", "fd2d")
-GPU = os.path.join(ROOT, "2d-vqsp-fd-sim_amd", "fortran", "_build", "fd2d_gpu")
+BUILD = os.path.join(ROOT, "2d-vqsp-fd-sim_amd", "fortran", "_build")
+# fd2d_gpu writes the receiver files a chunk of steps at a time; fd2d_gpu_refw
+# calls the reference's own writers step by step
+DROPINS = ("fd2d_gpu", "fd2d_gpu_refw")
 
 
-@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(GPU)),
-                    reason="reference build or drop-in binary not built")
+@pytest.mark.skipif(not (os.path.exists(REF) and all(os.path.exists(os.path.join(BUILD, d))
+                                                     for d in DROPINS)),
+                    reason="reference build or drop-in binaries not built")
+@pytest.mark.parametrize("dropin", DROPINS)
 @pytest.mark.parametrize("name", ["layered", "edges"])
-def test_dropin_outputs_byte_identical(name, tmp_path):
+def test_dropin_outputs_byte_identical(name, dropin, tmp_path):
     job = jobs.ALL[name]()
     job.key_tlv = job.key_tlq = job.key_tlp = job.key_snv = True
     job.ipas2 = 25
     outs = {}
-    for tag, exe in (("ref", REF), ("gpu", GPU)):
+    for tag, exe in (("ref", REF), ("gpu", os.path.join(BUILD, dropin))):
         d = tmp_path / tag
         formats.write_job(job, str(d))
-        print(f"running {tag} on {name}", flush=True)
+        print(f"running {tag} ({os.path.basename(exe)}) on {name}", flush=True)
         subprocess.run([exe], cwd=str(d), check=True, stdout=subprocess.DEVNULL, timeout=300)
         outs[tag] = d
     files = sorted(f for f in os.listdir(outs["ref"]) if f.startswith(("REC_", "SNAP.")))

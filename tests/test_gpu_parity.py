@@ -119,3 +119,24 @@ def test_kernel_timing_leaves_results_unchanged(every):
         np.testing.assert_array_equal(a.field(f), b.field(f))
     a.close()
     b.close()
+
+
+def test_set_receivers_replaced_before_first_step_rejected_after():
+    """A second bq_set_receivers before the first step replaces the receivers
+    (and frees the previous buffers); once a step has run it is rejected."""
+    job = jobs.tiny_layered(steps=30)
+    sim = lib.Simulation(job)
+    rec = np.asarray(job.receivers, np.int32).reshape(-1, 4)[:3]
+    cols = [np.ascontiguousarray(rec[:, k]) for k in range(4)]
+    sim._chk(sim.lib.bq_set_receivers(sim.h, 3, *[lib._ptr(c) for c in cols]))
+    sim.mr = 3
+    sim.run(job.mt2)
+    job3 = jobs.tiny_layered(steps=30)
+    job3.receivers = [tuple(r) for r in rec.tolist()]
+    ref = oracle.OracleSim(job3, "f32")
+    ref.run(job3.mt2)
+    np.testing.assert_array_equal(sim.traces(), ref.traces())
+    rc = sim.lib.bq_set_receivers(sim.h, 3, *[lib._ptr(c) for c in cols])
+    assert rc == -6
+    assert b"after the first step" in sim.lib.bq_last_error(sim.h)
+    np.testing.assert_array_equal(sim.traces(), ref.traces())

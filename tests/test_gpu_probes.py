@@ -1,19 +1,22 @@
-"""Bandwidth probes of the library (diagnostics; need a GPU)."""
+"""Bandwidth probes of the diagnostics library (need a GPU)."""
 import pytest
 
-from biotfd import lib
+from biotfd import lib, probes
 
 pytestmark = pytest.mark.gpu
 
 
 def test_copy_probe_reports_bandwidth():
-    assert lib.probe_copy(1 << 22, 3) > 0
+    assert probes.probe_copy(1 << 22, 3) > 0
 
 
-def test_march_probe_reports_bandwidth():
-    assert lib.probe_march(1100, 300, 3) > 0
+def test_march_probe_reports_bandwidth_and_exact_bytes():
+    gbs, moved = probes.probe_march(1100, 300, 3)
+    assert gbs > 0
+    # 2 x 512 columns, 2 blocks of 128 rows: 9 arrays read over 132 rows, 8 written over 128
+    assert moved == 1024 * 2 * 132 * 9 * 4 + 1024 * 2 * 128 * 8 * 4
 
 
 def test_march_probe_rejects_small_grids():
     with pytest.raises(lib.BiotError):
-        lib.probe_march(100, 300, 1)
+        probes.probe_march(100, 300, 1)

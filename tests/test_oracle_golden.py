@@ -86,3 +86,31 @@ def test_fp32_error_vs_fp64_oracle(name):
     for f, fname in enumerate(FIELDS):
         err = rel_l2(z["snaps"][-1, f], s64.field(fname))
         assert err < 1e-4, (fname, err)
+
+
+def test_oracle_full_cpu_runnable_workload_matches_reference_digests(tmp_path):
+    """The binary32 oracle on the CPU-runnable workload at its full size (400 x
+    400 cells, 1500 steps, 100 receivers) writes every snapshot and receiver
+    file byte for byte as the reference does (digests in cfg0_digests.npz)."""
+    import hashlib
+    from biotfd import hostsetup, writers
+    z = np.load(os.path.join(HERE, "golden", "cfg0_digests.npz"), allow_pickle=False)
+    want = dict(zip(z["files"].tolist(), z["sha256"].tolist()))
+    job = make_golden.cfg0_job()
+    sim = oracle.OracleSim(job, "f32")
+    inside = writers.receivers_inside(job.receivers, 1 - job.tpml, job.mx + job.tpml,
+                                      job.mzp + job.tpml)
+    w = writers.SeismogramWriter(str(tmp_path), job.dt, len(job.receivers), key_tlq=True,
+                                 key_tlp=True, inside=inside)
+    done = 0
+    for step in z["snap_steps"]:
+        step = int(step)
+        sim.run(step - done)
+        w.write_chunk(done + 1, sim.traces(done + 1, step - done))
+        done = step
+        writers.write_snapshots(str(tmp_path), step, sim.field("U"), sim.field("W"),
+                                sim.field("QX"), sim.field("QZ"))
+    assert len(want) == 3 * len(job.receivers) + 2 * len(z["snap_steps"])
+    for f, h in want.items():
+        with open(tmp_path / f, "rb") as fh:
+            assert hashlib.sha256(fh.read()).hexdigest() == h, f
