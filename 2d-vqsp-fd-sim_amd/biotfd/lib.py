@@ -24,7 +24,13 @@ EXPORTS = (
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells",
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
+    "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
+    "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
 )
+
+SCHED_GRID, SCHED_PERSISTENT = 0, 1
+# candidate c of bq_batch_autotune: tile height TUNE_ROWS[c // 2], schedule c % 2
+TUNE_ROWS = (64, 128, 256)
 
 
 class Desc(C.Structure):
@@ -79,6 +85,15 @@ def load(path: str = LIBPATH):
         "bq_write_seismograms": ([C.c_char_p, p, C.c_int, C.c_int, C.c_int, C.c_float, p, C.c_int,
                                   C.c_int, p], C.c_int),
         "bq_format_e13_6": ([C.c_float, C.c_char_p], C.c_int),
+        "bq_batch_create": ([p, C.c_int, C.POINTER(p)], C.c_int),
+        "bq_batch_destroy": ([p], None),
+        "bq_batch_run": ([p, C.c_int], C.c_int),
+        "bq_batch_sync": ([p], C.c_int),
+        "bq_batch_last_run_ms": ([p, p], C.c_int),
+        "bq_batch_autotune": ([p, C.c_int, p], C.c_int),
+        "bq_batch_set_schedule": ([p, C.c_int, C.c_int, C.c_int], C.c_int),
+        "bq_batch_get_schedule": ([p, p, p, p], C.c_int),
+        "bq_batch_last_error": ([p], C.c_char_p),
     }
     for name, (args, res) in sig.items():
         try:
@@ -224,6 +239,73 @@ class Simulation:
         if getattr(self, "h", None):
             self.lib.bq_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many :class:`Simulation` shots advanced together (``bq_batch_*``): three
+    launches per time step for all of them, replayed from a HIP graph. Each
+    shot's results are bit-identical to running it alone; read them from the
+    shots as usual."""
+
+    def __init__(self, sims):
+        self.lib = load()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        b = C.c_void_p()
+        rc = self.lib.bq_batch_create(arr, len(self.sims), C.byref(b))
+        if rc:
+            msg = self.lib.bq_batch_last_error(None)
+            raise BiotError(f"bq_batch_create failed ({rc}): {msg.decode() if msg else ''}")
+        self.b = b
+
+    def _chk(self, rc):
+        if rc:
+            msg = self.lib.bq_batch_last_error(self.b)
+            raise BiotError(f"biotfd batch error {rc}: {msg.decode() if msg else ''}")
+
+    def run(self, nsteps: int, sync: bool = True):
+        self._chk(self.lib.bq_batch_run(self.b, int(nsteps)))
+        for s in self.sims:
+            s.step += nsteps
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._chk(self.lib.bq_batch_sync(self.b))
+
+    def last_run_ms(self) -> float:
+        out = np.zeros(1, np.float64)
+        self._chk(self.lib.bq_batch_last_run_ms(self.b, _ptr(out)))
+        return float(out[0])
+
+    def autotune(self, trial_steps: int = 4) -> dict:
+        """Times the six candidates on the batch's own steps (6 * trial_steps
+        steps, which advance every shot) and keeps the fastest; returns
+        {(tile_rows, schedule): ms per step}."""
+        out = np.zeros(6, np.float64)
+        self._chk(self.lib.bq_batch_autotune(self.b, int(trial_steps), _ptr(out)))
+        for s in self.sims:
+            s.step += 6 * trial_steps
+        return {(TUNE_ROWS[c // 2], c % 2): float(out[c]) for c in range(6)}
+
+    def set_schedule(self, tile_rows: int = 128, schedule: int = SCHED_GRID, graph: bool = True):
+        self._chk(self.lib.bq_batch_set_schedule(self.b, int(tile_rows), int(schedule), int(graph)))
+
+    def schedule(self):
+        out = np.zeros(3, np.int32)
+        self._chk(self.lib.bq_batch_get_schedule(self.b, _ptr(out), _ptr(out[1:]), _ptr(out[2:])))
+        return int(out[0]), int(out[1]), bool(out[2])
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.bq_batch_destroy(self.b)
+            self.b = None
 
     def __del__(self):
         try:

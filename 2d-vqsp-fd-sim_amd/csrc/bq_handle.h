@@ -1,0 +1,49 @@
+// bq_handle.h - the state one grid (one shot) holds on the device, shared by
+// the single-grid entry points (bq_device.hip) and the batches (bq_batch.hip).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "bq_kernels.h"
+
+struct bq_handle {
+  GridDesc g{};
+  int P = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  KParams kp{};
+  std::vector<void *> allocs;
+  float *fields[2] = {nullptr, nullptr};  // two full field sets, current and next
+  int parity = 0;                         // steps done, mod 2
+  std::vector<int32_t> jm_host;
+  std::vector<float> tab_host;
+  bool model_ok = false, pml_ok = false, src_ok = false;
+  int next_step = 1;
+  int mr = 0;
+  int timing = 0;  // 0: off; n: events around the launches of every n-th step
+  std::vector<hipEvent_t> ev;
+  double ktime[3] = {0, 0, 0};
+  int timed_steps = 0;
+  Split ss{}, sv{};
+  Fused fu{};
+  long long fused_cells = 0, stress_cells = 0, velocity_cells = 0;
+  std::string err;
+};
+
+// Kernel parameters of a step of the given parity (steps done, mod 2): after
+// an odd number of steps the current and next field sets trade places.
+inline KParams params_for(const bq_handle *h, int parity) {
+  KParams k = h->kp;
+  if (parity) {
+    const size_t fsz = (size_t)h->P * h->g.NZ;
+    const float **f0[8] = {&k.U0, &k.W0, &k.QX0, &k.QZ0, &k.TXX0, &k.TZZ0, &k.TXZ0, &k.PR0};
+    float **f1[8] = {&k.U1, &k.W1, &k.QX1, &k.QZ1, &k.TXX1, &k.TZZ1, &k.TXZ1, &k.PR1};
+    for (int f = 0; f < 8; ++f) {
+      *f0[f] = h->fields[1] + f * fsz;
+      *f1[f] = h->fields[0] + f * fsz;
+    }
+  }
+  return k;
+}
+
+inline KParams step_params(const bq_handle *h) { return params_for(h, h->parity); }

@@ -147,6 +147,55 @@ int bq_source_from_stf(const float *t, const float *v, int nstf, int mt2, float 
 int bq_nonref(int key, float h, float dt, float vel, float vels, float velp, const float prm[4],
               float out[8]);
 
+/* ---- Batches: many shots advanced together ------------------------------
+ *
+ * The reference runs one shot per process and splits one grid over MPI ranks
+ * (domain_decomposition.F90:40-143); it has no batching. A batch groups the
+ * handles of independent shots - each a whole TIME_LOOP (time_loop.F90:65-206)
+ * of its own model, grid size, source and receivers - on one device and
+ * advances all of them with three launches per time step in total, replayed
+ * from a HIP graph. Every shot's fields and traces are bit-identical to a run
+ * of that shot alone with bq_run. The handles stay the shots' own: their
+ * results are read with bq_get_field / bq_get_traces as usual, and after a
+ * batch run they continue at their next step. While a batch runs, its shots
+ * must not be used otherwise. */
+typedef struct bq_batch bq_batch;
+
+/* Schedules of the batched launches: one block per work item, or a persistent
+ * grid of a few blocks per CU taking work items from per-XCD queues. */
+enum { BQ_SCHED_GRID = 0, BQ_SCHED_PERSISTENT = 1 };
+
+/* Groups nshots distinct handles of one device (each with model, PML and
+ * source set, receivers optional). The batch keeps the pointers; destroy it
+ * before its handles. */
+int bq_batch_create(bq_handle *const *shots, int nshots, bq_batch **out);
+
+/* Advances every shot by nsteps steps from its own next step (the shots may
+ * be at different steps; none may pass its MT2). Queued on the batch's stream;
+ * later calls on the shots' handles are ordered after it. */
+int bq_batch_run(bq_batch *b, int nsteps);
+int bq_batch_sync(bq_batch *b);
+
+/* Device time of the last bq_batch_run, ms (waits for it). */
+int bq_batch_last_run_ms(bq_batch *b, double *ms);
+
+/* Autotuner: runs trial_steps real steps of the batch with each of the six
+ * candidates - fused tile height 64, 128, 256 rows x schedule grid, persistent
+ * (candidate c: height index c / 2, schedule c % 2) - times each with HIP
+ * events and keeps the fastest. The 6*trial_steps steps advance the shots like
+ * any others (every candidate gives identical results). ms_per_step[6], if not
+ * NULL, receives each candidate's device time per step. */
+int bq_batch_autotune(bq_batch *b, int trial_steps, double *ms_per_step);
+
+/* The schedule by hand: tile_rows 64, 128 or 256, schedule BQ_SCHED_*,
+ * use_graph 0/1 (replay steps from a HIP graph). */
+int bq_batch_set_schedule(bq_batch *b, int tile_rows, int schedule, int use_graph);
+int bq_batch_get_schedule(bq_batch *b, int *tile_rows, int *schedule, int *use_graph);
+
+/* Message of the last failure; NULL for a failed bq_batch_create. */
+const char *bq_batch_last_error(bq_batch *b);
+void bq_batch_destroy(bq_batch *b);
+
 /* Output writers (host side, no handle). STORE_SEISMOGRAMS and
  * STORE_SEISMOGRAMS_PRESSURE (store_seismograms.F90:47-154,
  * store_seismograms_pressure.F90:45-83) for a chunk of steps: samples is

@@ -1,0 +1,84 @@
+#!/usr/bin/env python3
+"""States the binary32 error of the GPU time loop against the binary64 oracle
+(relative L2 of the last snapshot of each field and of each receiver column
+over all steps) on long runs, printing one JSON line per case:
+
+* the roofline model (8000 x 4000 physical cells, layered x correlated field,
+  quantized materials) on its full grid for --steps-full steps;
+* the same model statistics on a 1000 x 500 grid for its full 8000 steps
+  (--steps-long), so that the error growth over the workload's whole record
+  is stated at a size the binary64 oracle finishes in minutes.
+
+Needs a GPU; the oracle runs on the host cores (OMP_NUM_THREADS).
+
+    python tests/fp64_statement.py [--steps-full N] [--steps-long N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path[:0] = [os.path.join(ROOT, "2d-vqsp-fd-sim_amd"), os.path.join(ROOT, "oracle"), ROOT]
+
+import bench  # noqa: E402
+import oracle  # noqa: E402
+from biotfd import hostsetup, lib  # noqa: E402
+
+FIELDS = ("U", "W", "QX", "QZ", "TXX", "TZZ", "TXZ", "P")
+COLS = ("U", "-W", "QX", "-QZ", "PRES", "-TXX")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a))
+
+
+def state(name, job):
+    nz, nx = hostsetup.grid_shape(job)
+    t0 = time.perf_counter()
+    sim = lib.Simulation(job)
+    sim.run(job.mt2)
+    tg = time.perf_counter() - t0
+    print(f"[fp64] {name}: GPU {job.mt2} steps in {tg:.1f} s", file=sys.stderr, flush=True)
+    ref = oracle.OracleSim(job, "f64")
+    t0 = time.perf_counter()
+    left = job.mt2
+    while left:
+        n = min(left, 500)
+        ref.run(n)
+        left -= n
+        print(f"[fp64] {name}: oracle at step {job.mt2 - left} ({time.perf_counter() - t0:.0f} s)",
+              file=sys.stderr, flush=True)
+    out = {"case": name, "grid": [nx, nz], "steps": job.mt2, "receivers": len(job.receivers),
+           "jmnum": int(job.table.shape[0]),
+           "fields_rel_l2": {f: rel_l2(sim.field(f), ref.field(f)) for f in FIELDS}}
+    ta, tb = sim.traces(), ref.traces()
+    out["traces_rel_l2"] = {c: rel_l2(ta[:, :, k], tb[:, :, k]) for k, c in enumerate(COLS)}
+    out["oracle_s"] = round(time.perf_counter() - t0, 1)
+    print(json.dumps(out), flush=True)
+    sim.close()
+    ref.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps-full", type=int, default=600)
+    ap.add_argument("--steps-long", type=int, default=8000)
+    args = ap.parse_args()
+    if args.steps_long:
+        state("roofline model statistics, 1000x500 grid", bench.make_workload(args.steps_long, 1000, 500))
+    if args.steps_full:
+        state("roofline model, full 8000x4000 grid", bench.make_workload(args.steps_full))
+
+
+if __name__ == "__main__":
+    main()

@@ -100,8 +100,8 @@ def _cfg2_job(steps):
 
 
 @pytest.mark.parametrize("which,steps,start", [
-    ("layered 4000x2000", 24, "random"), ("layered 4000x2000", 40, "rest"),
-    ("roofline 8000x4000", 12, "random"), ("roofline 8000x4000", 24, "rest"),
+    ("layered 4000x2000", 24, "random"), ("layered 4000x2000", 300, "rest"),
+    ("roofline 8000x4000", 12, "random"), ("roofline 8000x4000", 120, "rest"),
 ])
 def test_full_grid_bitexact_vs_oracle(which, steps, start):
     t0 = time.perf_counter()
