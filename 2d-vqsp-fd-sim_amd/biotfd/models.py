@@ -250,3 +250,21 @@ def layered_job(name="LAYER", mx=4000, mz=2000, h=2.0, steps=5000, n_layers=10, 
     recs = receivers_row(n_rec, 1, step, 2)
     return make_job(name, table_rocks, jm, h, steps, f0, xs, 0.0, recs, dt=dt, tpml=tpml,
                     key_tlp=True, **kw)
+
+
+def shot_set(n_shots=64, steps=3000, seed=1000, mx_range=(500, 3000), mz_range=(500, 1500),
+             f0_range=(10.0, 40.0), h=2.0, tpml=20, n_rec=1000, n_layers=10) -> list:
+    """The batched-shots workload: ``n_shots`` layered half-spaces (as
+    :func:`layered_job`, shot s seeded with ``seed + s``) whose physical grid
+    sizes are drawn uniformly from ``mx_range`` x ``mz_range``, each with a
+    surface source at a random x and a Ricker frequency drawn from
+    ``f0_range``, surface receivers, and ``steps`` time steps."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_shots):
+        mx = int(rng.integers(mx_range[0], mx_range[1] + 1))
+        mz = int(rng.integers(mz_range[0], mz_range[1] + 1))
+        f0 = float(rng.uniform(*f0_range))
+        out.append(layered_job(name=f"SHOT{s:03d}", mx=mx, mz=mz, h=h, steps=steps,
+                               n_layers=n_layers, seed=seed + s, tpml=tpml, f0=f0, n_rec=n_rec))
+    return out

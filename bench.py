@@ -88,11 +88,118 @@ def make_like(job, steps):
     return j
 
 
+def _digest(*arrays) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def run_batched(args):
+    """Batched shots: N seeded layered half-spaces of sizes drawn from 500..3000 x
+    500..1500 cells, 20-cell PML, surface source with f0 ~ U(10, 40) Hz, up to
+    1000 surface receivers, warmup + steps time steps each. First every shot
+    alone in sequence (bq_run, the plain form), then all of them as one batch
+    (bq_batch_run), whose warmup steps are the autotuner's trials. Both time
+    exactly `steps` steps per shot with inputs resident; the batch's fields and
+    traces must equal the plain runs' bit for bit."""
+    total = args.warmup + args.steps
+    trial = max(1, args.warmup // 6)
+    if args.warmup < 6:
+        raise SystemExit("--warmup must be >= 6 for the batched workload (autotuner trials)")
+    t0 = time.perf_counter()
+    jobs = models.shot_set(args.shots, total, seed=1000)
+    cells = [int(np.prod(hostsetup.grid_shape(j))) for j in jobs]
+    print(f"[bench] {len(jobs)} shots ready, {sum(cells) / 1e6:.1f} M cells per step "
+          f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    # plain: one shot after another, each with its own three launches per step
+    per_shot, digests = [], []
+    for j in jobs:
+        sim = lib.Simulation(j)
+        sim.run(args.warmup)
+        a = time.perf_counter()
+        sim.run(args.steps)   # returns after bq_sync
+        per_shot.append(time.perf_counter() - a)
+        digests.append(_digest(sim.traces(), sim.field("U"), sim.field("P")))
+        sim.close()
+    t_plain = sum(per_shot)
+    print(f"[bench] plain: {t_plain:.2f} s for {len(jobs)} shots", file=sys.stderr, flush=True)
+    # batched
+    sims = [lib.Simulation(j) for j in jobs]
+    b = lib.Batch(sims)
+    tune = b.autotune(trial)
+    if 6 * trial < args.warmup:
+        b.run(args.warmup - 6 * trial)
+    b.sync()
+    a = time.perf_counter()
+    b.run(args.steps, sync=False)
+    b.sync()
+    t_batch = time.perf_counter() - a
+    dev_ms = b.last_run_ms()
+    rows, sched, graph = b.schedule()
+    same = all(_digest(s.traces(), s.field("U"), s.field("P")) == d for s, d in zip(sims, digests))
+    b.close()
+    for s in sims:
+        s.close()
+    if not same:
+        raise SystemExit("batched results differ from the shots run alone")
+    n = len(jobs)
+    step_cells = sum(cells)
+    gcell = step_cells * args.steps / t_batch / 1e9
+    out = {
+        "metric": "shots/s batched vs plain at <=1e-3 L2 misfit",
+        "value": round(n / t_batch, 3),
+        "unit": "shots/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} batched shots, layered half-spaces 500..3000 x 500..1500 "
+                               f"cells, dx=2 m, TPML=20, f0 U(10,40) Hz, surface receivers",
+                   "shots": n, "cells_per_step": step_cells, "receivers": sum(len(j.receivers) for j in jobs)},
+        "plain_shots_per_s": round(n / t_plain, 3),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "plain_shot_s": {"p50": round(float(np.percentile(per_shot, 50)), 4),
+                         "p99": round(float(np.percentile(per_shot, 99)), 4),
+                         "max": round(max(per_shot), 4)},
+        "batched_shot_s": {"p50": round(t_batch, 4), "p99": round(t_batch, 4),
+                           "note": "every shot of the batch completes with the batch"},
+        "l2_misfit_vs_plain": 0.0, "bit_identical_to_plain": True,
+        "batch_device_ms": round(dev_ms, 2),
+        "schedule": {"tile_rows": rows, "schedule": ["grid", "persistent"][sched], "hip_graph": graph},
+        "autotune_ms_per_step": {f"{r}/{['grid', 'persistent'][sc]}": round(v, 4)
+                                 for (r, sc), v in tune.items()},
+        "gcell_per_s": round(gcell, 2),
+        "roofline": {"bound": "hbm", "kernel": "whole batched step (kb_step, kb_velocity, kb_edges)",
+                     "basis": "104 B per cell-step over all shots' cells",
+                     "algorithmic_bytes": int(B_STEP * step_cells),
+                     "achieved": round(gcell * B_STEP, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gcell * B_STEP / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # the reference, serial, on the median-sized shot: per-step time from a
+        # 12- and a 2-step run, times the steps of a shot
+        med = jobs[int(np.argsort(cells)[len(cells) // 2])]
+        per_step = cpu_reference_baseline(med)
+        if per_step and per_step > 0:
+            out["cpu_baseline"] = {
+                "value": round(1.0 / (per_step * args.steps), 6), "unit": "shots/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"reference fd2d (serial) on the median shot {med.mx}x{med.mz}, 12 minus "
+                          f"2 steps, scaled to {args.steps} steps per shot"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=("roofline", "batched"), default="roofline",
+                    help="roofline: the 8000 x 4000 model (default); batched: many shots")
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--mx", type=int, default=8000)
     ap.add_argument("--mz", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -103,6 +210,12 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batched":
+        args.steps = 3000 if args.steps is None else args.steps
+        args.warmup = 12 if args.warmup is None else args.warmup
+        return run_batched(args)
+    args.steps = 2000 if args.steps is None else args.steps
+    args.warmup = 100 if args.warmup is None else args.warmup
 
     total = args.warmup + args.steps
     t0 = time.perf_counter()
