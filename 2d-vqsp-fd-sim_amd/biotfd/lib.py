@@ -26,6 +26,7 @@ EXPORTS = (
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
     "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
     "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
+    "bq_trace_misfit",
 )
 
 SCHED_GRID, SCHED_PERSISTENT = 0, 1
@@ -94,6 +95,7 @@ def load(path: str = LIBPATH):
         "bq_batch_set_schedule": ([p, C.c_int, C.c_int, C.c_int], C.c_int),
         "bq_batch_get_schedule": ([p, p, p, p], C.c_int),
         "bq_batch_last_error": ([p], C.c_char_p),
+        "bq_trace_misfit": ([p, p, C.c_int, C.c_int, p, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -245,6 +247,20 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+def trace_misfit(a: "Simulation", b: "Simulation", first: int = 1, n: int | None = None,
+                 per_receiver: bool = False, entry: str = "bq_trace_misfit"):
+    """Sums of the trace misfit of run a against run b on the device
+    (bq_trace_misfit): (sum (a-b)^2, sum b^2) over steps first .. first+n-1,
+    and per receiver [mr, 2] when per_receiver is set."""
+    n = min(a.step, b.step) - first + 1 if n is None else n
+    tot = np.zeros(2, np.float64)
+    per = np.zeros((a.mr, 2), np.float64) if per_receiver else None
+    rc = getattr(load(), entry)(a.h, b.h, int(first), int(n),
+                                _ptr(per) if per is not None else None, _ptr(tot))
+    a._chk(rc)
+    return (tot, per) if per_receiver else tot
 
 
 class Batch:

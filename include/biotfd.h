@@ -196,6 +196,18 @@ int bq_batch_get_schedule(bq_batch *b, int *tile_rows, int *schedule, int *use_g
 const char *bq_batch_last_error(bq_batch *b);
 void bq_batch_destroy(bq_batch *b);
 
+/* Misfit of two runs' receiver traces over steps first_step ..
+ * first_step+nsteps-1, computed on the device where the traces lie (no
+ * counterpart in the reference, which only writes its seismograms to files):
+ * total[0] = sum over those steps, the receivers and the six columns of
+ * bq_get_traces of (a - b)^2, total[1] = sum of b^2, accumulated in binary64;
+ * per_receiver[2j], [2j+1] (NULL: not returned) the same for receiver j.
+ * sqrt(total[0]/total[1]) is the relative L2 misfit of a against b. Both runs
+ * must have the same receiver count and have run those steps; b's queued work
+ * is waited for. Allocates a staging buffer of 14*mr+2 doubles per call. */
+int bq_trace_misfit(bq_handle *a, bq_handle *b, int first_step, int nsteps, double *per_receiver,
+                    double total[2]);
+
 /* Output writers (host side, no handle). STORE_SEISMOGRAMS and
  * STORE_SEISMOGRAMS_PRESSURE (store_seismograms.F90:47-154,
  * store_seismograms_pressure.F90:45-83) for a chunk of steps: samples is

@@ -103,3 +103,26 @@ def test_batch_rejects_steps_past_a_shots_end():
     b.run(5)
     with pytest.raises(lib.BiotError):
         lib.Batch([sims[0], sims[0]])
+
+
+def test_trace_misfit_on_device_matches_host():
+    """bq_trace_misfit against the same sums formed on the host from the traces:
+    two runs of one model with different PML widths and source frequencies
+    (the same receivers)."""
+    ja = jobs.medium_layered(steps=STEPS)
+    jb = jobs.medium_layered(steps=STEPS, f0=55.0)
+    jb.tpml = 16
+    a, b = lib.Simulation(ja), lib.Simulation(jb)
+    a.run(STEPS)
+    b.run(STEPS)
+    ta, tb = a.traces().astype(np.float64), b.traces().astype(np.float64)
+    for first, n in ((1, STEPS), (11, 20)):
+        tot, per = lib.trace_misfit(a, b, first, n, per_receiver=True)
+        da, db = ta[first - 1:first - 1 + n], tb[first - 1:first - 1 + n]
+        want_per = np.stack([((da - db) ** 2).sum(axis=(0, 2)), (db ** 2).sum(axis=(0, 2))], axis=1)
+        np.testing.assert_allclose(per, want_per, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(tot, want_per.sum(axis=0), rtol=1e-12, atol=0)
+        assert tot[0] > 0 and tot[1] > 0
+    assert lib.trace_misfit(a, a)[0] == 0.0
+    with pytest.raises(lib.BiotError, match="steps not run"):
+        lib.trace_misfit(a, b, 1, STEPS + 1)
