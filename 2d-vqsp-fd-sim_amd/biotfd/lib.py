@@ -26,7 +26,7 @@ EXPORTS = (
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
     "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
     "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
-    "bq_trace_misfit",
+    "bq_trace_misfit", "bq_trace_misfit_noalloc",
 )
 
 SCHED_GRID, SCHED_PERSISTENT = 0, 1
@@ -96,6 +96,7 @@ def load(path: str = LIBPATH):
         "bq_batch_get_schedule": ([p, p, p, p], C.c_int),
         "bq_batch_last_error": ([p], C.c_char_p),
         "bq_trace_misfit": ([p, p, C.c_int, C.c_int, p, p], C.c_int),
+        "bq_trace_misfit_noalloc": ([p, p, C.c_int, C.c_int, p, p], C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -252,7 +253,8 @@ class Simulation:
 def trace_misfit(a: "Simulation", b: "Simulation", first: int = 1, n: int | None = None,
                  per_receiver: bool = False, entry: str = "bq_trace_misfit"):
     """Sums of the trace misfit of run a against run b on the device
-    (bq_trace_misfit): (sum (a-b)^2, sum b^2) over steps first .. first+n-1,
+    (bq_trace_misfit; entry="bq_trace_misfit_noalloc" for the variant that
+    allocates nothing): (sum (a-b)^2, sum b^2) over steps first .. first+n-1,
     and per receiver [mr, 2] when per_receiver is set."""
     n = min(a.step, b.step) - first + 1 if n is None else n
     tot = np.zeros(2, np.float64)

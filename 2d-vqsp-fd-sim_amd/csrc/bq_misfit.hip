@@ -6,10 +6,14 @@
 // sum (a - b)^2 and sum b^2, per receiver and in total, accumulated in
 // binary64. The traces stay where the runs left them, in HBM.
 //
-// bq_trace_misfit stages per-column partial sums in a buffer it allocates on
-// each call.
+// Two entry points share the kernels. bq_trace_misfit stages per-column
+// partial sums in a buffer it allocates on each call. bq_trace_misfit_noalloc
+// stages them in a file-scope device array instead and allocates nothing, for
+// callers that cannot allocate (code under HIP graph capture, tight serving
+// loops); it takes at most kMisfitMaxReceivers receivers.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/biotfd.h"
@@ -18,6 +22,15 @@
 namespace {
 
 constexpr int kMisfitBlock = 256;
+constexpr int kMisfitMaxReceivers = 8192;
+// staging of one call: 12*mr column sums, 2*mr receiver sums, 2 totals
+constexpr int kMisfitStage = 14 * kMisfitMaxReceivers + 2;
+
+// Staging of bq_trace_misfit_noalloc (917,520 bytes): one copy per device,
+// held until the process ends and shared by every call on that device. Host
+// code never passes this symbol to a kernel; it uses the address that
+// hipGetSymbolAddress returned for the device.
+static __device__ __align__(256) double scratch[kMisfitStage];
 
 // Thread t = j*6 + c (receiver j, column c) sums its column over the steps:
 // st[2t] = sum (a - b)^2, st[2t+1] = sum b^2. a and b are [nsteps][mr][6],
@@ -118,6 +131,30 @@ int misfit_run(bq_handle *a, bq_handle *b, int first_step, int nsteps, double *s
   return 0;
 }
 
+// Address of each device's copy of `scratch`, looked up once per device with
+// that device current.
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for(bq_handle *h, double **out) {
+  if (h->device < 0 || h->device >= kMaxDevices) return mfail(h, -2, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[h->device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != h->device) e = hipSetDevice(h->device);
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != h->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return mfail(h, -100, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[h->device] = (double *)p;
+  }
+  *out = g_scratch[h->device];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -133,6 +170,20 @@ int bq_trace_misfit(bq_handle *a, bq_handle *b, int first_step, int nsteps, doub
   const int rc = misfit_run(a, b, first_step, nsteps, stage, per_receiver, total);
   (void)hipFree(stage);
   return rc;
+}
+
+// Every value the kernels read from the staging array is written earlier in
+// the same call (column sums, then receiver sums and totals), so nothing has
+// to be cleared between calls. Calls that overlap on one device race on it.
+int bq_trace_misfit_noalloc(bq_handle *a, bq_handle *b, int first_step, int nsteps,
+                            double *per_receiver, double total[2]) {
+  if (!total) return mfail(a, -1, "bq_trace_misfit_noalloc: null output");
+  if (int rc = misfit_check(a, b, first_step, nsteps)) return rc;
+  if (a->mr > kMisfitMaxReceivers)
+    return mfail(a, -3, "bq_trace_misfit_noalloc: more than 8192 receivers; use bq_trace_misfit");
+  double *stage = nullptr;
+  if (int rc = scratch_for(a, &stage)) return rc;
+  return misfit_run(a, b, first_step, nsteps, stage, per_receiver, total);
 }
 
 }  // extern "C"

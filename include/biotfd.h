@@ -208,6 +208,15 @@ void bq_batch_destroy(bq_batch *b);
 int bq_trace_misfit(bq_handle *a, bq_handle *b, int first_step, int nsteps, double *per_receiver,
                     double total[2]);
 
+/* bq_trace_misfit without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops). Stages in a fixed device array of 917,520
+ * bytes per device (held until the process ends), so it takes at most 8192
+ * receivers (-3 beyond); calls that overlap on one device race on that
+ * array. Same results as bq_trace_misfit. */
+int bq_trace_misfit_noalloc(bq_handle *a, bq_handle *b, int first_step, int nsteps,
+                            double *per_receiver, double total[2]);
+
 /* Output writers (host side, no handle). STORE_SEISMOGRAMS and
  * STORE_SEISMOGRAMS_PRESSURE (store_seismograms.F90:47-154,
  * store_seismograms_pressure.F90:45-83) for a chunk of steps: samples is
