@@ -19,8 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BIOTFD_ARCH", "gfx950")
 OUT = os.path.join(HERE, "biotfd", "libbiotfd.so")
 SOURCES = [os.path.join(HERE, "csrc", f)
-           for f in ("bq_device.hip", "bq_batch.hip", "bq_misfit.hip", "bq_host.cpp",
-                     "bq_output.cpp")]
+           for f in ("bq_device.hip", "bq_batch.hip", "bq_misfit.hip", "bq_learned.hip",
+                     "bq_host.cpp", "bq_output.cpp")]
 PROBES_OUT = os.path.join(HERE, "biotfd", "libbiotfd_probes.so")
 PROBES_SOURCES = [os.path.join(HERE, "csrc", "bq_probes.hip")]
 

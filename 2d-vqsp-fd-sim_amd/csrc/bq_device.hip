@@ -546,6 +546,7 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     if (e) (void)hipEventRecord(e[2], h->stream);
     hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
     if (e) (void)hipEventRecord(e[3], h->stream);
+    if (h->lb && bq::lb_after_step(h, k, it)) return fail(h, -100, "learned boundary launch failed");
     if (it == itile_end && k.MR > 0)
       hipLaunchKernelGGL(k_sample, dim3(nsb), dim3(kBlock), 0, h->stream, k, it);
   }

@@ -27,8 +27,23 @@ struct bq_handle {
   Split ss{}, sv{};
   Fused fu{};
   long long fused_cells = 0, stress_cells = 0, velocity_cells = 0;
+  // learned absorbing boundary (bq_lb_attach): the network, and the band of
+  // the last four time levels, level n in slot n % 4
+  const void *lb = nullptr;
+  float *lb_ring = nullptr;
   std::string err;
 };
+
+inline int fail_handle(bq_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+namespace bq {
+// Strip cells from the attached network after step `it`, whose parameters
+// are k (bq_learned.hip).
+int lb_after_step(bq_handle *h, const KParams &k, int it);
+}  // namespace bq
 
 // Kernel parameters of a step of the given parity (steps done, mod 2): after
 // an odd number of steps the current and next field sets trade places.

@@ -1,0 +1,197 @@
+// bq_learned.h - device code of the learned absorbing boundary, shared by the
+// single-grid launches (bq_learned.hip) and the batches (bq_batch.hip).
+//
+// A grid whose PML is a thin strip of S = 8 cells on the left, right and bottom
+// (the reference's CPML of ppml.F90:159-306 with TPML = 8) has its strip cells
+// replaced after every step by a small network that stands in for a wide PML.
+// For each side the network sees a band of 12 cells across the boundary - the
+// 8 strip cells and the 4 interior cells next to them - for all 8 fields at
+// the last 4 time levels, and predicts the change of the 8 strip cells' 8
+// fields over the step:
+//
+//   input  x[p][c]  p along the boundary, c = level*96 + field*12 + cell
+//                   (384 channels, each field divided by its scale)
+//   layer1 h1 = lrelu(conv5(x,  W1: 384 -> 64))
+//   layer2 h2 = lrelu(conv5(h1, W2:  64 -> 64))
+//   layer3 d  =       conv5(h2, W3:  64 -> 64)     d[p][field*8 + cell]
+//   strip(t+1) = strip(t) + scale[field] * d
+//
+// conv5 is a 5-tap convolution along the boundary with zero padding at the
+// ends of each layer's input, no bias; lrelu(y) = max(y, 0.1 y). Inference
+// runs the three layers in BF16 on the matrix cores (MFMA 16x16x16, FP32
+// accumulation), all three in LDS for a tile of 52 positions per block.
+//
+// Band layout (one time level): side 0 (left, x = -7..4, rows l = 0..MZ'),
+// side 1 (right, x = MX+8..MX-3, rows 0..MZ'), side 2 (bottom, rows
+// l = MZ'+8..MZ'-3, x = -7..MX+8), each [pos][field][cell] with cell 0 the
+// outermost strip cell, 7 the innermost, 8..11 the interior cells inward.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bq_kernels.h"
+
+namespace bqk {
+
+constexpr int kLbS = 8;                 // strip cells per side
+constexpr int kLbCells = 12;            // band cells across the boundary
+constexpr int kLbF = 8;                 // fields
+constexpr int kLbBand = kLbF * kLbCells;  // 96 values per band position and level
+constexpr int kLbLev = 4;               // input time levels
+constexpr int kLbCin = kLbLev * kLbBand;  // 384
+constexpr int kLbH = 64;                // hidden channels = outputs (8 fields x 8 cells)
+constexpr int kLbTap = 5;               // taps of each convolution
+constexpr int kLbP = 52;                // valid output positions per inference block
+constexpr int kLbRows = 68;             // LDS rows of each activation tile
+constexpr int kLbXS = kLbCin + 8;       // LDS row stride of the input tile (bf16)
+constexpr int kLbHS = kLbH + 8;         // LDS row stride of the hidden tiles (bf16)
+constexpr int kLbBlock = 256;
+
+// Weights of the three networks in BF16, [tap][out][in] (in contiguous), and
+// the per-field scales of each side.
+struct LbNet {
+  const unsigned short *w1[3], *w2[3], *w3[3];
+  float scale[3][kLbF], inv[3][kLbF];
+};
+
+__host__ __device__ inline int lb_lz(int MZ) { return MZ + 1; }
+__host__ __device__ inline int lb_lx(int MX) { return MX + 2 * kLbS; }
+__host__ __device__ inline int lb_side_len(int side, int MX, int MZ) {
+  return side < 2 ? lb_lz(MZ) : lb_lx(MX);
+}
+__host__ __device__ inline size_t lb_side_off(int side, int MZ) {
+  return (size_t)side * lb_lz(MZ) * kLbBand;
+}
+__host__ __device__ inline size_t lb_band_size(int MX, int MZ) {
+  return (size_t)(2 * lb_lz(MZ) + lb_lx(MX)) * kLbBand;
+}
+
+// Grid cell (i, l) of band position pos, cell c of a side.
+__device__ __forceinline__ void lb_cell(int side, int pos, int c, int MX, int MZ, int *i, int *l) {
+  if (side == 0) {
+    *i = 1 - kLbS + c;
+    *l = pos;
+  } else if (side == 1) {
+    *i = MX + kLbS - c;
+    *l = pos;
+  } else {
+    *i = 1 - kLbS + pos;
+    *l = MZ + kLbS - c;
+  }
+}
+
+// Element e of the band of one level, from the fields F[8] of a grid with
+// pitch P and column offset OFF (element (i,l) at l*P + i + OFF).
+__device__ __forceinline__ void lb_extract(float *__restrict__ out, const float *const *F, int P,
+                                           int OFF, int MX, int MZ, size_t e) {
+  const size_t s1 = lb_side_off(1, MZ), s2 = lb_side_off(2, MZ);
+  const int side = e < s1 ? 0 : (e < s2 ? 1 : 2);
+  const size_t r = e - lb_side_off(side, MZ);
+  const int pos = (int)(r / kLbBand), fc = (int)(r % kLbBand);
+  const int f = fc / kLbCells, c = fc % kLbCells;
+  int i, l;
+  lb_cell(side, pos, c, MX, MZ, &i, &l);
+  out[e] = F[f][(size_t)l * P + (size_t)(i + OFF)];
+}
+
+__device__ __forceinline__ unsigned short lb_bf16(float x) {
+  // round to nearest even (the values are finite)
+  unsigned int u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+typedef short lb_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float lb_f32x4 __attribute__((ext_vector_type(4)));
+
+struct LbLds {
+  unsigned short x[kLbRows * kLbXS];
+  unsigned short h1[kLbRows * kLbHS];
+  unsigned short h2[kLbRows * kLbHS];
+};
+
+// One 5-tap convolution layer on the matrix cores for a tile of 64 output
+// rows: out[r][co] = sum_t sum_ci in[r + t][ci] * w[t][co][ci], in and out in
+// LDS (bf16, row strides is / os), w in global memory. Wave v computes rows
+// 16v..16v+15 for all 64 outputs. The MFMA 16x16x16 operand layout: lane q
+// holds A[row q % 16][k 4(q/16) .. +3] and B[k 4(q/16) .. +3][col q % 16];
+// the result holds D[row 4(q/16) + j][col q % 16], j = 0..3.
+template <int CIN>
+__device__ __forceinline__ void lb_layer(const unsigned short *in, int is,
+                                         const unsigned short *__restrict__ w, lb_f32x4 acc[4]) {
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  const int qr = lane & 15, qk = 4 * (lane >> 4);
+  for (int n = 0; n < 4; ++n) acc[n] = lb_f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < kLbTap; ++t) {
+    const unsigned short *arow = in + (size_t)(16 * v + qr + t) * is + qk;
+    const unsigned short *wt = w + (size_t)t * kLbH * CIN + qk;
+#pragma unroll 4
+    for (int cb = 0; cb < CIN; cb += 16) {
+      const lb_bf16x4 a = *(const lb_bf16x4 *)(arow + cb);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        const lb_bf16x4 b = *(const lb_bf16x4 *)(wt + (size_t)(16 * n + qr) * CIN + cb);
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Hidden layer result -> LDS (bf16), lrelu, zero outside the side's
+// positions (row r is at position base + r).
+__device__ __forceinline__ void lb_store_hidden(unsigned short *h, const lb_f32x4 acc[4], int base,
+                                                int L) {
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  for (int n = 0; n < 4; ++n)
+    for (int j = 0; j < 4; ++j) {
+      const int r = 16 * v + 4 * (lane >> 4) + j, co = 16 * n + (lane & 15);
+      const int p = base + r;
+      float y = acc[n][j];
+      y = y > 0.f ? y : 0.1f * y;
+      h[r * kLbHS + co] = (p >= 0 && p < L) ? lb_bf16(y) : (unsigned short)0;
+    }
+  // rows 64..67 only feed rows whose results are discarded
+  for (int e = threadIdx.x; e < 4 * kLbHS; e += kLbBlock) h[64 * kLbHS + e] = 0;
+}
+
+// One tile of one side: positions p0 .. p0+51. lev[0..3]: the band at the
+// last four time levels, oldest first (level t = lev[3]). The change of the
+// strip cells is added to lev[3]'s strip values and written by put(p, f, c,
+// value).
+template <class Put>
+__device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_t soff, int p0,
+                                        const float *const *lev, LbLds &S, Put put) {
+  const float *inv = net.inv[side];
+  // input rows r at positions p0 - 6 + r
+  for (int e = threadIdx.x; e < kLbRows * kLbCin; e += kLbBlock) {
+    const int r = e / kLbCin, ci = e % kLbCin;
+    const int p = p0 - 6 + r;
+    const int lv = ci / kLbBand, fc = ci % kLbBand;
+    float x = 0.f;
+    if (p >= 0 && p < L) x = lev[lv][soff + (size_t)p * kLbBand + fc] * inv[fc / kLbCells];
+    S.x[r * kLbXS + ci] = lb_bf16(x);
+  }
+  __syncthreads();
+  lb_f32x4 acc[4];
+  lb_layer<kLbCin>(S.x, kLbXS, net.w1[side], acc);
+  lb_store_hidden(S.h1, acc, p0 - 4, L);
+  __syncthreads();
+  lb_layer<kLbH>(S.h1, kLbHS, net.w2[side], acc);
+  lb_store_hidden(S.h2, acc, p0 - 2, L);
+  __syncthreads();
+  lb_layer<kLbH>(S.h2, kLbHS, net.w3[side], acc);
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  for (int n = 0; n < 4; ++n)
+    for (int j = 0; j < 4; ++j) {
+      const int r = 16 * v + 4 * (lane >> 4) + j, co = 16 * n + (lane & 15);
+      const int p = p0 + r;
+      if (r < kLbP && p < L) {
+        const int f = co >> 3, c = co & 7;
+        const float prev = lev[3][soff + (size_t)p * kLbBand + f * kLbCells + c];
+        put(p, f, c, prev + net.scale[side][f] * acc[n][j]);
+      }
+    }
+}
+
+__host__ __device__ inline int lb_tiles(int L) { return (L + kLbP - 1) / kLbP; }
+
+}  // namespace bqk

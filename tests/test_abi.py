@@ -12,8 +12,8 @@ from biotfd import lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    src = open(os.path.join(ROOT, "include", "biotfd.h")).read()
+def declared_functions(header="biotfd.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(bq_[a-z_0-9]+)\s*\(", src)))
 
@@ -89,3 +89,14 @@ def test_probes_library_is_separate():
         assert hasattr(so, n), n
     assert not set(names) & set(declared_functions())
     assert not any(hasattr(lib.load(), n) for n in names)
+
+
+def test_learned_boundary_entry_points_are_exported():
+    """The learned-boundary extension (include/biotfd_learned.h) is part of the
+    time-loop library and bound by biotfd.learned."""
+    from biotfd import learned
+    names = declared_functions("biotfd_learned.h")
+    assert sorted(names) == sorted(learned.EXPORTS)
+    so = lib.load()
+    for n in names:
+        assert hasattr(so, n), n
