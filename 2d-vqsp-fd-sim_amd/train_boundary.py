@@ -1,0 +1,102 @@
+#!/usr/bin/env python3
+"""Trains the learned absorbing boundary on one MI355X and checks it.
+
+1. Records bands from --models seeded layered models of 1000 x 1000 cells
+   (dx = 2 m, random source position, Ricker 10..40 Hz) run with a 40-cell
+   PML, in batches: groups of five consecutive levels every --every steps
+   from --first on, over --steps steps.
+2. Scales per side and field (RMS), then Adam on each side's network
+   (--iters minibatches of 8 groups, cosine-decayed learning rate --lr),
+   with a tenth of the groups held out for the validation loss.
+3. Runs --check held-out models (seeds from 10000) three ways - 40-cell PML
+   (the judge), 8-cell strip with the network, 8-cell plain PML - and prints
+   the reflected energy of the last two against the first at a few levels.
+
+Writes the networks to --out (npz: weights, scales, loss history, check).
+
+    python 2d-vqsp-fd-sim_amd/train_boundary.py --out weights.npz
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from biotfd import learned, lib  # noqa: E402
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def check(net, n, steps, every, size=1000):
+    """Reflected energy of the learned and the plain 8-cell boundary against
+    the 40-cell PML run on n held-out models."""
+    out = []
+    for k in range(n):
+        seed = 10_000 + k
+        wide = lib.Simulation(learned.training_job(seed, steps, 40, size, size))
+        lrn = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
+        pln = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
+        net.attach(lrn)
+        acc = {"learned": np.zeros(2), "plain8": np.zeros(2)}
+        done = 0
+        for t in range(every, steps + 1, every):
+            for s in (wide, lrn, pln):
+                s.run(t - done)
+            done = t
+            acc["learned"] += learned.reflected_energy(lrn, wide)
+            acc["plain8"] += learned.reflected_energy(pln, wide)
+        r = {m: float(v[0] / v[1]) for m, v in acc.items()}
+        log(f"[lb] check model {seed}: reflected energy learned {r['learned']:.3e} "
+            f"plain8 {r['plain8']:.3e}")
+        out.append(r)
+        for s in (wide, lrn, pln):
+            s.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--models", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--first", type=int, default=300)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--check", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="lb_strip8.npz")
+    args = ap.parse_args()
+    t0 = time.perf_counter()
+    per_model = len(range(args.first, args.steps - 1, args.every))
+    tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch)
+    ng = learned.record(tr, args.models, args.steps, args.first, args.every, log=log)
+    t_rec = time.perf_counter() - t0
+    scales = tr.compute_scales(ng)
+    log(f"[lb] {ng} groups in {t_rec:.1f} s; scales {np.array2string(scales, precision=3)}")
+    t1 = time.perf_counter()
+    hist = learned.train(tr, learned.init_weights(args.seed), args.iters, args.lr, seed=args.seed, log=log)
+    t_train = time.perf_counter() - t1
+    net = learned.Network(tr.get_weights(), scales)
+    res = check(net, args.check, args.steps, 500) if args.check else []
+    summary = {"groups": ng, "record_s": round(t_rec, 1), "train_s": round(t_train, 1),
+               "iters": args.iters, "lr": args.lr,
+               "final_train_loss": [h[-1] for h in hist["train"]],
+               "final_val_loss": [h[-1] for h in hist["val"]],
+               "check": res}
+    net.save(args.out, train_loss=np.array(hist["train"]), val_loss=np.array(hist["val"]),
+             info=json.dumps(summary))
+    print(json.dumps(summary), flush=True)
+
+
+if __name__ == "__main__":
+    main()
