@@ -51,14 +51,17 @@ def _lib():
         sig = {
             "bq_band_size": ([C.c_int, C.c_int], C.c_int),
             "bq_band_extract": ([p, p], C.c_int),
-            "bq_lb_create": ([p, p, C.POINTER(p)], C.c_int),
+            "bq_lb_create": ([p, p, C.c_int, C.c_float, C.POINTER(p)], C.c_int),
+            "bq_strip_copy": ([p, p], C.c_int),
+            "bq_lb_trainer_capture_residual": ([p, p, p, C.c_int], C.c_int),
             "bq_lb_destroy": ([p], None),
             "bq_lb_last_error": ([], C.c_char_p),
             "bq_lb_attach": ([p, p], C.c_int),
             "bq_lb_detach": ([p], C.c_int),
             "bq_lb_infer": ([p, C.c_int, C.c_int, p, p], C.c_int),
             "bq_field_misfit": ([p, p, C.c_int, p], C.c_int),
-            "bq_lb_trainer_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(p)], C.c_int),
+            "bq_lb_trainer_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(p)],
+                                     C.c_int),
             "bq_lb_trainer_destroy": ([p], None),
             "bq_lb_trainer_last_error": ([p], C.c_char_p),
             "bq_lb_trainer_capture": ([p, p, C.c_int, C.c_int], C.c_int),
@@ -77,7 +80,8 @@ def _lib():
 
 
 EXPORTS = ("bq_band_size", "bq_band_extract", "bq_lb_create", "bq_lb_destroy", "bq_lb_last_error",
-           "bq_lb_attach", "bq_lb_detach", "bq_lb_infer", "bq_field_misfit", "bq_lb_trainer_create",
+           "bq_lb_attach", "bq_lb_detach", "bq_lb_infer", "bq_field_misfit", "bq_strip_copy",
+           "bq_lb_trainer_capture_residual", "bq_lb_trainer_create",
            "bq_lb_trainer_destroy", "bq_lb_trainer_last_error", "bq_lb_trainer_capture",
            "bq_lb_trainer_sync", "bq_lb_trainer_scales", "bq_lb_trainer_set_weights",
            "bq_lb_trainer_get_weights", "bq_lb_trainer_step")
@@ -106,10 +110,11 @@ def split_weights(w_side: np.ndarray):
     return w1, w2, w3
 
 
-def torch_forward(w_side: np.ndarray, scale_side: np.ndarray, bands: np.ndarray) -> np.ndarray:
+def torch_forward(w_side: np.ndarray, scale_side: np.ndarray, bands: np.ndarray,
+                  gain: float = 1.0) -> np.ndarray:
     """The network of one side in FP32 (torch, CPU): bands [4][L][96] -> the
-    new strip values [L][8][8] (field, cell). The reference the device
-    inference is checked against."""
+    newest level's strip values plus gain * scale * output, [L][8][8] (field,
+    cell). The reference the device inference is checked against."""
     import torch
     x, inv = _inputs(bands, scale_side)
     ws = [torch.from_numpy(np.ascontiguousarray(w.transpose(2, 1, 0))) for w in split_weights(w_side)]
@@ -120,7 +125,7 @@ def torch_forward(w_side: np.ndarray, scale_side: np.ndarray, bands: np.ndarray)
             h = torch.nn.functional.leaky_relu(h, 0.1)
     d = h[0].T.numpy().reshape(-1, F, S)                          # [L][field][cell]
     prev = bands[3].reshape(-1, F, CELLS)[:, :, :S]
-    return (prev + scale_side[None, :, None] * d).astype(np.float32)
+    return (prev + np.float32(gain) * scale_side[None, :, None] * d).astype(np.float32)
 
 
 def _inputs(bands, scale_side):
@@ -131,28 +136,38 @@ def _inputs(bands, scale_side):
 
 
 class Network:
-    """Weights [3][NSIDE] and scales [3][8] of the three sides; on the device
-    as a ``bq_lb`` for :meth:`attach`."""
+    """Weights [3][NSIDE] and scales [3][8] of the three sides, the form
+    (residual: correcting the 8-cell CPML's update, or direct) and the output
+    gain; on the device as a ``bq_lb`` for :meth:`attach`."""
 
-    def __init__(self, weights: np.ndarray, scales: np.ndarray):
+    def __init__(self, weights: np.ndarray, scales: np.ndarray, residual: bool = True,
+                 gain: float = 1.0):
         self.weights = np.ascontiguousarray(weights, np.float32).reshape(3, NSIDE)
         self.scales = np.ascontiguousarray(scales, np.float32).reshape(3, F)
+        self.residual = bool(residual)
+        self.gain = float(gain)
         self._h = None
 
     @classmethod
-    def load(cls, path: str = WEIGHTS) -> "Network":
+    def load(cls, path: str = WEIGHTS, gain: float | None = None) -> "Network":
         z = np.load(path, allow_pickle=False)
-        return cls(z["weights"], z["scales"])
+        g = float(z["gain"]) if gain is None and "gain" in z else (1.0 if gain is None else gain)
+        return cls(z["weights"], z["scales"], bool(z["residual"]) if "residual" in z else True, g)
+
+    def with_gain(self, gain: float) -> "Network":
+        return Network(self.weights, self.scales, self.residual, gain)
 
     def save(self, path: str, **info):
         os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
         np.savez_compressed(path, weights=self.weights, scales=self.scales,
+                            residual=np.asarray(self.residual), gain=np.asarray(self.gain),
                             **{k: np.asarray(v) for k, v in info.items()})
 
     def device(self):
         if self._h is None:
             h = C.c_void_p()
-            rc = _lib().bq_lb_create(_ptr(self.weights), _ptr(self.scales), C.byref(h))
+            rc = _lib().bq_lb_create(_ptr(self.weights), _ptr(self.scales), int(self.residual),
+                                     float(self.gain), C.byref(h))
             if rc:
                 raise BiotError(f"bq_lb_create failed ({rc}): {_lib().bq_lb_last_error().decode()}")
             self._h = h
@@ -204,6 +219,11 @@ def band_host(fields: dict, mx: int, mzp: int, tpml: int) -> np.ndarray:
     return np.concatenate(out).astype(np.float32)
 
 
+def strip_copy(dst: Simulation, src: Simulation):
+    """The 8-cell strip of dst's current fields from src's (bq_strip_copy)."""
+    dst._chk(_lib().bq_strip_copy(dst.h, src.h))
+
+
 def field_misfit(a: Simulation, b: Simulation, field: str) -> np.ndarray:
     """(sum (a-b)^2, sum b^2) of one field over the common physical cells."""
     out = np.zeros(2, np.float64)
@@ -216,13 +236,14 @@ def field_misfit(a: Simulation, b: Simulation, field: str) -> np.ndarray:
 class Trainer:
     """Band store and FP32 training kernels on the device (bq_lb_trainer)."""
 
-    def __init__(self, mx: int, mzp: int, max_groups: int, batch: int = 8):
+    def __init__(self, mx: int, mzp: int, max_groups: int, batch: int = 8, residual: bool = True):
         h = C.c_void_p()
-        rc = _lib().bq_lb_trainer_create(mx, mzp, max_groups, batch, C.byref(h))
+        rc = _lib().bq_lb_trainer_create(mx, mzp, max_groups, batch, int(residual), C.byref(h))
         if rc:
             raise BiotError(f"bq_lb_trainer_create failed ({rc}): "
                             f"{_lib().bq_lb_trainer_last_error(None).decode()}")
         self.t, self.mx, self.mzp, self.max_groups, self.batch = h, mx, mzp, max_groups, batch
+        self.residual = bool(residual)
         self.ngroups = 0
         self.scales = np.ones((3, F), np.float32)
 
@@ -232,6 +253,9 @@ class Trainer:
 
     def capture(self, sim: Simulation, group: int, level: int):
         self._chk(_lib().bq_lb_trainer_capture(self.t, sim.h, group, level))
+
+    def capture_residual(self, wide: Simulation, trunc: Simulation, group: int):
+        self._chk(_lib().bq_lb_trainer_capture_residual(self.t, wide.h, trunc.h, group))
 
     def compute_scales(self, ngroups: int) -> np.ndarray:
         out = np.zeros((3, F), np.float32)
@@ -283,36 +307,62 @@ def training_job(seed: int, steps: int, tpml: int = 40, mx: int = 1000, mz: int 
 def record(trainer: Trainer, n_models: int, steps: int, first: int, every: int,
            per_batch: int = 50, log=print) -> int:
     """Runs n_models wide-PML (TPML = 40) training models, per_batch at a time
-    as one batch, and records groups of five levels t-3 .. t+1 at t = first,
-    first + every, ... < steps. Returns the number of groups."""
+    as one batch, and records one group per model at t = first, first + every,
+    ... < steps - 1: levels t-3 .. t of the wide run as input and the target
+    level. Residual trainer: every model also runs on an 8-cell grid whose
+    strip is copied from the wide run after every step, so that it stays on
+    the wide run's trajectory; level 4 is the wide band at t+1 minus that
+    grid's band after its own step t+1 (before the copy). Direct trainer:
+    level 4 is the wide band at t+1. Returns the number of groups."""
     times = list(range(first, steps - 1, every))
-    g = 0
+    # steps at which each level is taken: t-3, t-2, t-1, t (inputs), t+1
+    when = {}
+    for gi, t in enumerate(times):
+        for lev in range(5):
+            when.setdefault(t - 3 + lev, []).append((gi, lev))
+    g0 = 0
     for b0 in range(0, n_models, per_batch):
-        jobs = [training_job(s, steps) for s in range(b0, min(n_models, b0 + per_batch))]
-        sims = [Simulation(j) for j in jobs]
-        batch = lib.Batch(sims)
-        done = 0
+        seeds = list(range(b0, min(n_models, b0 + per_batch)))
+        if g0 + len(seeds) * len(times) > trainer.max_groups:
+            break
+        wide = [Simulation(training_job(s, steps, 40)) for s in seeds]
+        bw = lib.Batch(wide)
+        trunc = bt = None
+        if trainer.residual:
+            trunc = [Simulation(training_job(s, steps, 8)) for s in seeds]
+            bt = lib.Batch(trunc)
         t0 = time.perf_counter()
-        for t in times:
-            if g + len(sims) > trainer.max_groups:
-                break
-            batch.run(t - 3 - done)
-            done = t - 3
-            for lev in range(5):
-                if lev:
-                    batch.run(1)
-                    done += 1
-                for k, s in enumerate(sims):
-                    trainer.capture(s, g + k, lev)
-            g += len(sims)
+        last = max(when)
+        done = 0
+        step = 0
+        while step < last:
+            # without a truncated grid, run straight to the next recorded level
+            nxt = step + 1 if trainer.residual else min(k for k in when if k > step)
+            bw.run(nxt - step)
+            if bt is not None:
+                bt.run(nxt - step)
+            step = nxt
+            for gi, lev in when.get(step, []):
+                for k in range(len(seeds)):
+                    grp = g0 + gi * len(seeds) + k
+                    if lev == 4 and trainer.residual:
+                        trainer.capture_residual(wide[k], trunc[k], grp)
+                    else:
+                        trainer.capture(wide[k], grp, lev)
+            if trunc is not None:
+                for k in range(len(seeds)):
+                    strip_copy(trunc[k], wide[k])
         _lib().bq_lb_trainer_sync(trainer.t)
-        log(f"[lb] recorded models {b0}..{b0 + len(sims) - 1}: {g} groups "
+        g0 += len(seeds) * len(times)
+        log(f"[lb] recorded models {seeds[0]}..{seeds[-1]}: {g0} groups "
             f"({time.perf_counter() - t0:.1f} s)")
-        batch.close()
-        for s in sims:
+        for b in (bw, bt):
+            if b is not None:
+                b.close()
+        for s in wide + (trunc or []):
             s.close()
-    trainer.ngroups = g
-    return g
+    trainer.ngroups = g0
+    return g0
 
 
 def train(trainer: Trainer, weights: np.ndarray, iters: int, lr: float, val_frac: float = 0.1,

@@ -14,7 +14,12 @@
 //   layer1 h1 = lrelu(conv5(x,  W1: 384 -> 64))
 //   layer2 h2 = lrelu(conv5(h1, W2:  64 -> 64))
 //   layer3 d  =       conv5(h2, W3:  64 -> 64)     d[p][field*8 + cell]
-//   strip(t+1) = strip(t) + scale[field] * d
+//   strip(t+1) = strip(t) + gain * scale[field] * d            (direct)
+//   strip(t+1) = cpml(t+1) + gain * scale[field] * d           (residual)
+//
+// In the direct form the network is the strip's whole update; in the residual
+// form it corrects the update the 8-cell CPML has just made, and gain 0 is
+// the plain CPML.
 //
 // conv5 is a 5-tap convolution along the boundary with zero padding at the
 // ends of each layer's input, no bias; lrelu(y) = max(y, 0.1 y). Inference
@@ -51,6 +56,8 @@ constexpr int kLbBlock = 256;
 struct LbNet {
   const unsigned short *w1[3], *w2[3], *w3[3];
   float scale[3][kLbF], inv[3][kLbF];
+  int residual;   // 0: direct form, 1: residual form (see above)
+  float gain;
 };
 
 __host__ __device__ inline int lb_lz(int MZ) { return MZ + 1; }
@@ -80,9 +87,10 @@ __device__ __forceinline__ void lb_cell(int side, int pos, int c, int MX, int MZ
 }
 
 // Element e of the band of one level, from the fields F[8] of a grid with
-// pitch P and column offset OFF (element (i,l) at l*P + i + OFF).
+// pitch P and column offset OFF (element (i,l) at l*P + i + OFF); with sub,
+// subtracted from what out holds.
 __device__ __forceinline__ void lb_extract(float *__restrict__ out, const float *const *F, int P,
-                                           int OFF, int MX, int MZ, size_t e) {
+                                           int OFF, int MX, int MZ, size_t e, int sub) {
   const size_t s1 = lb_side_off(1, MZ), s2 = lb_side_off(2, MZ);
   const int side = e < s1 ? 0 : (e < s2 ? 1 : 2);
   const size_t r = e - lb_side_off(side, MZ);
@@ -90,7 +98,8 @@ __device__ __forceinline__ void lb_extract(float *__restrict__ out, const float 
   const int f = fc / kLbCells, c = fc % kLbCells;
   int i, l;
   lb_cell(side, pos, c, MX, MZ, &i, &l);
-  out[e] = F[f][(size_t)l * P + (size_t)(i + OFF)];
+  const float v = F[f][(size_t)l * P + (size_t)(i + OFF)];
+  out[e] = sub ? out[e] - v : v;
 }
 
 __device__ __forceinline__ unsigned short lb_bf16(float x) {
@@ -154,9 +163,9 @@ __device__ __forceinline__ void lb_store_hidden(unsigned short *h, const lb_f32x
 }
 
 // One tile of one side: positions p0 .. p0+51. lev[0..3]: the band at the
-// last four time levels, oldest first (level t = lev[3]). The change of the
-// strip cells is added to lev[3]'s strip values and written by put(p, f, c,
-// value).
+// last four time levels, oldest first (level t = lev[3]). For each strip
+// cell put(p, f, c, prev, d) receives lev[3]'s value and the network's
+// gain * scale * output.
 template <class Put>
 __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_t soff, int p0,
                                         const float *const *lev, LbLds &S, Put put) {
@@ -187,7 +196,7 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
       if (r < kLbP && p < L) {
         const int f = co >> 3, c = co & 7;
         const float prev = lev[3][soff + (size_t)p * kLbBand + f * kLbCells + c];
-        put(p, f, c, prev + net.scale[side][f] * acc[n][j]);
+        put(p, f, c, prev, net.gain * net.scale[side][f] * acc[n][j]);
       }
     }
 }

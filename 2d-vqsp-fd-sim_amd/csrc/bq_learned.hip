@@ -45,9 +45,26 @@ F8 fields_of(const KParams &k, bool next) {
 }
 
 __global__ __launch_bounds__(kTB) void k_lb_extract(F8 F, int P, int OFF, int MX, int MZ,
-                                                    float *__restrict__ out, size_t n) {
+                                                    float *__restrict__ out, size_t n, int sub) {
   for (size_t e = (size_t)blockIdx.x * kTB + threadIdx.x; e < n; e += (size_t)gridDim.x * kTB)
-    lb_extract(out, F.f, P, OFF, MX, MZ, e);
+    lb_extract(out, F.f, P, OFF, MX, MZ, e, sub);
+}
+
+// The strip cells (band cells 0..7 of the three sides, all fields) of grid
+// a's fields from grid b's, at the same physical cells.
+__global__ __launch_bounds__(kTB) void k_strip_copy(F8 A, int pa, int offa, F8 B, int pb, int offb,
+                                                    int MX, int MZ) {
+  const size_t lz = lb_lz(MZ), lx = lb_lx(MX);
+  const size_t n = (2 * lz + lx) * kLbF * kLbS;
+  for (size_t e = (size_t)blockIdx.x * kTB + threadIdx.x; e < n; e += (size_t)gridDim.x * kTB) {
+    const size_t pos_all = e / (kLbF * kLbS);
+    const int fc = (int)(e % (kLbF * kLbS)), f = fc / kLbS, c = fc % kLbS;
+    const int side = pos_all < lz ? 0 : (pos_all < 2 * lz ? 1 : 2);
+    const int pos = (int)(pos_all - (side == 0 ? 0 : side == 1 ? lz : 2 * lz));
+    int i, l;
+    lb_cell(side, pos, c, MX, MZ, &i, &l);
+    ((float *)A.f[f])[(size_t)l * pa + (size_t)(i + offa)] = B.f[f][(size_t)l * pb + (size_t)(i + offb)];
+  }
 }
 
 // lev[j] = the band at level t-3+j
@@ -66,12 +83,14 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, Lev
   const int tile = b - (side == 0 ? 0 : side == 1 ? t0 : t1);
   const int L = lb_side_len(side, MX, MZ);
   lb_tile(net, side, L, lb_side_off(side, MZ), tile * kLbP, lev.p, S,
-          [&](int p, int f, int c, float v) {
+          [&](int p, int f, int c, float prev, float d) {
             int i, l;
             lb_cell(side, p, c, MX, MZ, &i, &l);
             float *F = f == 0 ? k.U1 : f == 1 ? k.W1 : f == 2 ? k.QX1 : f == 3 ? k.QZ1
                      : f == 4 ? k.TXX1 : f == 5 ? k.TZZ1 : f == 6 ? k.TXZ1 : k.PR1;
-            F[(size_t)l * k.P + (size_t)(i + k.OFF)] = v;
+            float &x = F[(size_t)l * k.P + (size_t)(i + k.OFF)];
+            // residual: the CPML's update of this step is in x
+            x = (net.residual ? x : prev) + d;
           });
 }
 
@@ -80,8 +99,9 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, Lev
 __global__ __launch_bounds__(kLbBlock) void k_lb_infer(LbNet net, int side, int L, Lev4 lev,
                                                        float *__restrict__ out) {
   __shared__ LbLds S;
-  lb_tile(net, side, L, 0, blockIdx.x * kLbP, lev.p, S,
-          [&](int p, int f, int c, float v) { out[((size_t)p * kLbF + f) * kLbS + c] = v; });
+  lb_tile(net, side, L, 0, blockIdx.x * kLbP, lev.p, S, [&](int p, int f, int c, float prev, float d) {
+    out[((size_t)p * kLbF + f) * kLbS + c] = prev + d;
+  });
 }
 
 // ------------------------------------------------------------- training
@@ -90,10 +110,10 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_infer(LbNet net, int side, int 
 
 // Minibatch of one side from the recorded groups: x[b][p][lev*96 + fc] =
 // band level lev / scale, y[b][p][f*8 + c] = (strip at level 4 - level 3) /
-// scale.
+// scale, or with residual strip at level 4 / scale.
 __global__ __launch_bounds__(kTB) void k_tr_gather(const float *__restrict__ store, size_t gsize,
                                                    size_t bsize, size_t soff, const int *__restrict__ groups,
-                                                   int B, int L, const float *__restrict__ inv,
+                                                   int B, int L, const float *__restrict__ inv, int residual,
                                                    float *__restrict__ x, float *__restrict__ y) {
   const size_t nx = (size_t)B * L * kLbCin, ny = (size_t)B * L * kLbH;
   for (size_t e = (size_t)blockIdx.x * kTB + threadIdx.x; e < nx + ny; e += (size_t)gridDim.x * kTB) {
@@ -111,7 +131,8 @@ __global__ __launch_bounds__(kTB) void k_tr_gather(const float *__restrict__ sto
       const int b = (int)(bp / L), p = (int)(bp % L);
       const int f = co >> 3, c = co & 7;
       const float *g = store + (size_t)groups[b] * gsize + soff + (size_t)p * kLbBand + f * kLbCells + c;
-      y[q] = (g[4 * bsize] - g[3 * bsize]) * inv[f];
+      // residual: level 4 already holds the target (wide run - CPML update)
+      y[q] = (residual ? g[4 * bsize] : g[4 * bsize] - g[3 * bsize]) * inv[f];
     }
   }
 }
@@ -314,6 +335,7 @@ struct bq_lb_trainer {
   double *part = nullptr;
   int nparts = 0;
   int adam_t[3] = {0, 0, 0};
+  int residual = 0;
   float scale[3][kLbF];
   std::string err;
 };
@@ -331,7 +353,7 @@ int lb_after_step(bq_handle *h, const KParams &k, int it) {
   hipLaunchKernelGGL(k_lb_apply, dim3(tiles), dim3(kLbBlock), 0, h->stream, lb->net, k, lev);
   const int blocks = (int)std::min<size_t>((bs + kTB - 1) / kTB, 4096);
   hipLaunchKernelGGL(k_lb_extract, dim3(blocks), dim3(kTB), 0, h->stream, fields_of(k, true), k.P,
-                     k.OFF, k.MX, k.MZ, h->lb_ring + (size_t)(it & 3) * bs, bs);
+                     k.OFF, k.MX, k.MZ, h->lb_ring + (size_t)(it & 3) * bs, bs, 0);
   return hipGetLastError() == hipSuccess ? 0 : -100;
 }
 }  // namespace bq
@@ -357,8 +379,12 @@ int grid_of(size_t n) { return (int)std::min<size_t>((n + kTB - 1) / kTB, 1 << 2
 
 extern "C" {
 
-int bq_lb_create(const float *weights, const float *scales, bq_lb **out) {
+int bq_lb_create(const float *weights, const float *scales, int residual, float gain, bq_lb **out) {
   if (!out || !weights || !scales) return -1;
+  if (!(gain >= 0.f && gain <= 16.f)) {
+    g_lb_err = "bq_lb_create: gain outside [0, 16]";
+    return -2;
+  }
   *out = nullptr;
   bq_lb *lb = new bq_lb();
   (void)hipGetDevice(&lb->device);
@@ -384,6 +410,8 @@ int bq_lb_create(const float *weights, const float *scales, bq_lb **out) {
     bq_lb_destroy(lb);
     return -101;
   }
+  lb->net.residual = residual != 0;
+  lb->net.gain = gain;
   for (int s = 0; s < 3; ++s) {
     const unsigned short *base = lb->w + (size_t)s * kNSide;
     lb->net.w1[s] = base;
@@ -430,7 +458,7 @@ int bq_lb_attach(bq_handle *h, bq_lb *lb) {
   // level 0 (the fields before the first step) into slot 0
   const KParams k = step_params(h);
   hipLaunchKernelGGL(k_lb_extract, dim3(grid_of(bs)), dim3(kTB), 0, h->stream, fields_of(k, false),
-                     k.P, k.OFF, k.MX, k.MZ, h->lb_ring, bs);
+                     k.P, k.OFF, k.MX, k.MZ, h->lb_ring, bs, 0);
   if (hipStreamSynchronize(h->stream) != hipSuccess)
     return fail_handle(h, -100, "bq_lb_attach: band extraction failed");
   h->lb = lb;
@@ -445,14 +473,31 @@ int bq_lb_detach(bq_handle *h) {
 
 int bq_band_size(int mx, int mzp) { return mx < 8 || mzp < 6 ? -1 : (int)lb_band_size(mx, mzp); }
 
-int bq_band_extract(bq_handle *h, float *dst) {
+static int band_extract(bq_handle *h, float *dst, int sub) {
   if (!h || !dst) return -1;
   if (h->g.T < kLbS) return fail_handle(h, -2, "bq_band_extract: the grid's TPML must be >= 8");
   const KParams k = step_params(h);  // the current set holds the latest fields
   const size_t bs = lb_band_size(k.MX, k.MZ);
   hipLaunchKernelGGL(k_lb_extract, dim3(grid_of(bs)), dim3(kTB), 0, h->stream, fields_of(k, false),
-                     k.P, k.OFF, k.MX, k.MZ, dst, bs);
+                     k.P, k.OFF, k.MX, k.MZ, dst, bs, sub);
   if (hipGetLastError() != hipSuccess) return fail_handle(h, -100, "bq_band_extract: launch failed");
+  return 0;
+}
+
+int bq_band_extract(bq_handle *h, float *dst) { return band_extract(h, dst, 0); }
+
+int bq_strip_copy(bq_handle *dst, bq_handle *src) {
+  if (!dst || !src) return -1;
+  if (dst->g.MX != src->g.MX || dst->g.MZ != src->g.MZ)
+    return fail_handle(dst, -2, "bq_strip_copy: grids of different physical size");
+  if (dst->g.T < kLbS || src->g.T < kLbS)
+    return fail_handle(dst, -2, "bq_strip_copy: TPML must be >= 8");
+  if (hipStreamSynchronize(src->stream) != hipSuccess) return fail_handle(dst, -100, "sync failed");
+  const KParams kd = step_params(dst), ks = step_params(src);
+  const size_t n = (2 * (size_t)lb_lz(kd.MZ) + lb_lx(kd.MX)) * kLbF * kLbS;
+  hipLaunchKernelGGL(k_strip_copy, dim3(grid_of(n)), dim3(kTB), 0, dst->stream, fields_of(kd, false),
+                     kd.P, kd.OFF, fields_of(ks, false), ks.P, ks.OFF, kd.MX, kd.MZ);
+  if (hipGetLastError() != hipSuccess) return fail_handle(dst, -100, "bq_strip_copy: launch failed");
   return 0;
 }
 
@@ -507,10 +552,12 @@ int bq_field_misfit(bq_handle *a, bq_handle *b, int field_id, double out[2]) {
 
 // ---------------------------------------------------------------- trainer
 
-int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, bq_lb_trainer **out) {
+int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual,
+                         bq_lb_trainer **out) {
   if (!out || mx < 8 || mzp < 6 || max_groups < 1 || batch < 1) return -1;
   *out = nullptr;
   bq_lb_trainer *t = new bq_lb_trainer();
+  t->residual = residual != 0;
   (void)hipGetDevice(&t->device);
   t->MX = mx;
   t->MZ = mzp;
@@ -564,6 +611,18 @@ int bq_lb_trainer_capture(bq_lb_trainer *t, bq_handle *h, int group, int level) 
     return tfail(t, -2, "bq_lb_trainer_capture: grid of another physical size");
   if (int rc = bq_band_extract(h, t->store + (size_t)group * t->gsize + (size_t)level * t->bsize))
     return tfail(t, rc, h->err);
+  return 0;
+}
+
+int bq_lb_trainer_capture_residual(bq_lb_trainer *t, bq_handle *wide, bq_handle *trunc, int group) {
+  if (!t || !wide || !trunc || group < 0 || group >= t->max_groups) return -1;
+  if (wide->g.MX != t->MX || wide->g.MZ != t->MZ || trunc->g.MX != t->MX || trunc->g.MZ != t->MZ)
+    return tfail(t, -2, "bq_lb_trainer_capture_residual: grid of another physical size");
+  float *dst = t->store + (size_t)group * t->gsize + 4 * t->bsize;
+  if (int rc = band_extract(wide, dst, 0)) return tfail(t, rc, wide->err);
+  // the truncated grid's stream must see the wide band written
+  if (hipStreamSynchronize(wide->stream) != hipSuccess) return tfail(t, -100, "sync failed");
+  if (int rc = band_extract(trunc, dst, 1)) return tfail(t, rc, trunc->err);
   return 0;
 }
 
@@ -635,7 +694,7 @@ int bq_lb_trainer_step(bq_lb_trainer *t, int side, const int32_t *groups, float 
   float *w1 = t->w[side], *w2 = w1 + kNW1, *w3 = w2 + kNW2;
   hipLaunchKernelGGL(k_tr_gather, dim3(grid_of(nbl * (kLbCin + kLbH))), dim3(kTB), 0, st, t->store,
                      t->gsize, t->bsize, lb_side_off(side, t->MZ), t->groups, B, L, t->inv + side * kLbF,
-                     t->x, t->y);
+                     t->residual, t->x, t->y);
   const int go = grid_of(nout);
   hipLaunchKernelGGL(k_tr_fwd, dim3(go), dim3(kTB), 0, st, t->x, w1, t->y1, t->a1, B, L, kLbCin, 1);
   hipLaunchKernelGGL(k_tr_fwd, dim3(go), dim3(kTB), 0, st, t->a1, w2, t->y2, t->a2, B, L, kLbH, 1);

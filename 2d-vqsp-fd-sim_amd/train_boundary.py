@@ -36,30 +36,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def check(net, n, steps, every, size=1000):
-    """Reflected energy of the learned and the plain 8-cell boundary against
-    the 40-cell PML run on n held-out models."""
+def check(net, gains, n, steps, every, size=1000):
+    """Reflected energy of the learned boundary at each gain and of the plain
+    8-cell boundary against the 40-cell PML run on n held-out models."""
     out = []
     for k in range(n):
         seed = 10_000 + k
         wide = lib.Simulation(learned.training_job(seed, steps, 40, size, size))
-        lrn = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
-        pln = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
-        net.attach(lrn)
-        acc = {"learned": np.zeros(2), "plain8": np.zeros(2)}
+        runs = {"plain8": lib.Simulation(learned.training_job(seed, steps, 8, size, size))}
+        nets = []
+        for g in gains:
+            s = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
+            nets.append(net.with_gain(g))
+            nets[-1].attach(s)
+            runs[f"learned@{g:g}"] = s
+        acc = {m: np.zeros(2) for m in runs}
         done = 0
         for t in range(every, steps + 1, every):
-            for s in (wide, lrn, pln):
+            for s in [wide] + list(runs.values()):
                 s.run(t - done)
             done = t
-            acc["learned"] += learned.reflected_energy(lrn, wide)
-            acc["plain8"] += learned.reflected_energy(pln, wide)
+            for m, s in runs.items():
+                acc[m] += learned.reflected_energy(s, wide)
         r = {m: float(v[0] / v[1]) for m, v in acc.items()}
-        log(f"[lb] check model {seed}: reflected energy learned {r['learned']:.3e} "
-            f"plain8 {r['plain8']:.3e}")
+        log(f"[lb] check model {seed}: " + ", ".join(f"{m} {v:.3e}" for m, v in r.items()))
         out.append(r)
-        for s in (wide, lrn, pln):
+        for s in [wide] + list(runs.values()):
             s.close()
+        for x in nets:
+            x.close()
     return out
 
 
@@ -75,10 +80,16 @@ def main():
     ap.add_argument("--check", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="lb_strip8.npz")
+    ap.add_argument("--direct", action="store_true",
+                    help="train the direct form (the network is the strip's whole update) "
+                         "instead of the residual form (a correction of the 8-cell CPML)")
+    ap.add_argument("--gains", default="1,0.5,0.25",
+                    help="output gains to check on the held-out models")
     args = ap.parse_args()
     t0 = time.perf_counter()
     per_model = len(range(args.first, args.steps - 1, args.every))
-    tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch)
+    tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch,
+                         residual=not args.direct)
     ng = learned.record(tr, args.models, args.steps, args.first, args.every, log=log)
     t_rec = time.perf_counter() - t0
     scales = tr.compute_scales(ng)
@@ -86,8 +97,9 @@ def main():
     t1 = time.perf_counter()
     hist = learned.train(tr, learned.init_weights(args.seed), args.iters, args.lr, seed=args.seed, log=log)
     t_train = time.perf_counter() - t1
-    net = learned.Network(tr.get_weights(), scales)
-    res = check(net, args.check, args.steps, 500) if args.check else []
+    net = learned.Network(tr.get_weights(), scales, residual=not args.direct)
+    gains = [float(g) for g in args.gains.split(",")]
+    res = check(net, gains, args.check, args.steps, 500) if args.check else []
     summary = {"groups": ng, "record_s": round(t_rec, 1), "train_s": round(t_train, 1),
                "iters": args.iters, "lr": args.lr,
                "final_train_loss": [h[-1] for h in hist["train"]],

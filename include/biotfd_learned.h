@@ -26,7 +26,10 @@
  * fastest; tap t multiplies the input at position p + t - 2. Input channel
  * level*96 + field*12 + cell (level 0 = oldest), output channel field*8 +
  * cell. Scales: [3][8], one per side and field; inputs are divided by them and
- * the outputs (the change of the strip over one step) multiplied.
+ * the outputs multiplied. Two forms: direct (the output times gain is the
+ * strip's change over the step) and residual (the output times gain corrects
+ * the value the 8-cell CPML has just computed for the strip cell; gain 0 is
+ * the plain CPML).
  */
 #ifndef BIOTFD_LEARNED_H
 #define BIOTFD_LEARNED_H
@@ -50,8 +53,9 @@ int bq_band_size(int mx, int mzp);
 int bq_band_extract(bq_handle *h, float *dst);
 
 /* Networks of the three sides from FP32 weights and scales (layout above),
- * converted to BF16 on the device that is current. */
-int bq_lb_create(const float *weights, const float *scales, bq_lb **out);
+ * converted to BF16 on the device that is current; residual 0/1 selects the
+ * form, gain (0..16) scales the output. */
+int bq_lb_create(const float *weights, const float *scales, int residual, float gain, bq_lb **out);
 void bq_lb_destroy(bq_lb *lb);
 const char *bq_lb_last_error(void);
 
@@ -62,9 +66,14 @@ int bq_lb_attach(bq_handle *h, bq_lb *lb);
 int bq_lb_detach(bq_handle *h);
 
 /* The inference kernel on given bands of one side (diagnostics and tests):
- * bands [4][len][96] (oldest level first), out [len][8][8] the new strip
- * values (field, cell). */
+ * bands [4][len][96] (oldest level first), out [len][8][8] (field, cell) the
+ * newest level's strip values plus gain * scale * the network's output. */
 int bq_lb_infer(bq_lb *lb, int side, int len, const float *bands, float *out);
+
+/* The 8-cell strip (all fields) of dst's current fields from src's, at the
+ * same physical cells (same MX, MZ'; both TPML >= 8). Used to keep a
+ * truncated grid on the trajectory of a wide-PML run while recording. */
+int bq_strip_copy(bq_handle *dst, bq_handle *src);
 
 /* Misfit of one field (BQ_U ..) between two grids of the same physical size
  * over their physical cells x = 1..MX, rows 0..MZ' (their PML widths may
@@ -72,14 +81,21 @@ int bq_lb_infer(bq_lb *lb, int side, int len, const float *bands, float *out);
 int bq_field_misfit(bq_handle *a, bq_handle *b, int field_id, double out[2]);
 
 /* ---- trainer: bands of wide-PML runs, fitted on the device ---------------
- * Groups of five consecutive time levels t-3 .. t+1 of one run; levels 0..3
- * are the input, the strip's change from level 3 to 4 the target. All runs
- * recorded must have the trainer's MX, MZ' and TPML >= 8. */
-int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, bq_lb_trainer **out);
+ * Groups of five levels of one run; levels 0..3 (t-3 .. t) are the input.
+ * Direct form: level 4 is level t+1 and the target the strip's change from
+ * level 3 to 4. Residual form: level 4 is the wide run's band at t+1 minus
+ * the band of an 8-cell grid kept on its trajectory (bq_strip_copy) after
+ * that grid's own step t+1 (bq_lb_trainer_capture_residual), and the target
+ * its strip cells. All runs recorded must have the trainer's MX, MZ' and
+ * TPML >= 8. */
+int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual,
+                         bq_lb_trainer **out);
 void bq_lb_trainer_destroy(bq_lb_trainer *t);
 const char *bq_lb_trainer_last_error(bq_lb_trainer *t);
 /* Band of h's current fields as level `level` (0..4) of group `group`. */
 int bq_lb_trainer_capture(bq_lb_trainer *t, bq_handle *h, int group, int level);
+/* Level 4 of group `group`: band(wide) - band(trunc), both current fields. */
+int bq_lb_trainer_capture_residual(bq_lb_trainer *t, bq_handle *wide, bq_handle *trunc, int group);
 int bq_lb_trainer_sync(bq_lb_trainer *t);
 /* RMS per side and field over levels 0..3 of groups 0..ngroups-1: the scales
  * (written to scales[3][8] and used from then on). */

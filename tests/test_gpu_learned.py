@@ -57,10 +57,11 @@ def _host_fields(sim):
     return {f: sim.field(f) for f in FIELDS}
 
 
-def test_band_extract_matches_host_and_training_kernels_match_torch():
+@pytest.mark.parametrize("residual", [False, True])
+def test_band_extract_matches_host_and_training_kernels_match_torch(residual):
     import torch
     job = jobs.tiny_layered(steps=30, tpml=8)
-    tr = learned.Trainer(job.mx, job.mzp, max_groups=4, batch=2)
+    tr = learned.Trainer(job.mx, job.mzp, max_groups=4, batch=2, residual=residual)
     host = []   # [group][level] bands on the host
     for g in range(4):
         sim = lib.Simulation(job)
@@ -93,7 +94,8 @@ def test_band_extract_matches_host_and_training_kernels_match_torch():
         for g in groups:
             lv = host[g, :, offs[s]:offs[s] + lens[s] * learned.BAND].reshape(5, lens[s], 8, learned.CELLS)
             x = (lv[:4] * inv[None, None, :, None]).transpose(1, 0, 2, 3).reshape(lens[s], -1)
-            y = ((lv[4, :, :, :8] - lv[3, :, :, :8]) * inv[None, :, None]).reshape(lens[s], -1)
+            tgt = lv[4, :, :, :8] if residual else lv[4, :, :, :8] - lv[3, :, :, :8]
+            y = (tgt * inv[None, :, None]).reshape(lens[s], -1)
             xs.append(x.T)
             ys.append(y.T)
         x = torch.tensor(np.stack(xs), dtype=torch.float32)
@@ -127,10 +129,12 @@ def test_band_extract_matches_host_and_training_kernels_match_torch():
     tr.close()
 
 
-def test_attached_network_replaces_only_the_strip():
+@pytest.mark.parametrize("residual,gain", [(False, 1.0), (True, 0.5)])
+def test_attached_network_replaces_only_the_strip(residual, gain):
     """One step from a random state: every cell outside the 8-cell strip is
     bit-identical to the plain 8-cell PML run; the strip cells are the
-    network's prediction from the band of the start level."""
+    network's prediction from the band of the start level (direct form), or
+    the plain run's strip plus the network's correction (residual form)."""
     job = jobs.tiny_layered(steps=10, tpml=8)
     plain, run = lib.Simulation(job), lib.Simulation(job)
     _random_state(plain, 5)
@@ -138,7 +142,7 @@ def test_attached_network_replaces_only_the_strip():
     band0 = learned.band_host(_host_fields(run), job.mx, job.mzp, job.tpml)
     w = _weights(3, last=10.0)
     scales = np.full((3, 8), 0.7, np.float32)
-    net = learned.Network(w, scales)
+    net = learned.Network(w, scales, residual=residual, gain=gain)
     net.attach(run)
     plain.run(1)
     run.run(1)
@@ -151,7 +155,8 @@ def test_attached_network_replaces_only_the_strip():
         for s in range(3):
             bands = np.zeros((4, lens[s], learned.BAND), np.float32)
             bands[3] = band0[offs[s]:offs[s] + lens[s] * learned.BAND].reshape(lens[s], -1)
-            want = learned.torch_forward(w[s], scales[s], bands)[:, f, :]     # [L][8]
+            pred = learned.torch_forward(w[s], scales[s], bands, gain)[:, f, :]     # [L][8]
+            prev = bands[3].reshape(lens[s], 8, -1)[:, f, :8]
             c = np.arange(8)
             if s == 0:
                 ii, ll = (1 - 8 + c)[None, :], np.arange(lz)[:, None]
@@ -159,10 +164,50 @@ def test_attached_network_replaces_only_the_strip():
                 ii, ll = (job.mx + 8 - c)[None, :], np.arange(lz)[:, None]
             else:
                 ii, ll = (1 - 8 + np.arange(lx))[:, None], (job.mzp + 8 - c)[None, :]
-            ii, ll = np.broadcast_to(ii, want.shape), np.broadcast_to(ll, want.shape)
+            ii, ll = np.broadcast_to(ii, pred.shape), np.broadcast_to(ll, pred.shape)
             strip[ll, ii - pmll] = True
             got = a[ll, ii - pmll]
-            scale = np.abs(want - bands[3].reshape(lens[s], 8, -1)[:, f, :8]).max() + 1e-30
+            want = b[ll, ii - pmll] + (pred - prev) if residual else pred
+            scale = np.abs(pred - prev).max() + 1e-30
             assert np.abs(got - want).max() <= 2e-2 * scale + 1e-6 * np.abs(want).max(), (name, s)
         np.testing.assert_array_equal(a[~strip], b[~strip], err_msg=name)
     net.close()
+
+
+def test_strip_copy_and_residual_capture():
+    """bq_strip_copy puts a wide-PML run's values into an 8-cell grid's strip
+    (bit for bit, same physical cells); bq_lb_trainer_capture_residual records
+    band(wide) - band(8-cell grid), whose strip cells are the residual
+    trainer's target (checked through the loss of a zero network)."""
+    job_w = jobs.tiny_layered(steps=30, tpml=14)
+    job_t = jobs.tiny_layered(steps=30, tpml=8)
+    wide, trunc = lib.Simulation(job_w), lib.Simulation(job_t)
+    wide.run(6)
+    trunc.run(6)
+    learned.strip_copy(trunc, wide)
+    fw, ft = _host_fields(wide), _host_fields(trunc)
+    bw = learned.band_host(fw, job_w.mx, job_w.mzp, job_w.tpml)
+    bt = learned.band_host(ft, job_t.mx, job_t.mzp, job_t.tpml)
+    lz, lx = job_t.mzp + 1, job_t.mx + 2 * learned.S
+    views = [bw[:2 * lz * 96].reshape(2 * lz, 8, 12), bt[:2 * lz * 96].reshape(2 * lz, 8, 12),
+             bw[2 * lz * 96:].reshape(lx, 8, 12), bt[2 * lz * 96:].reshape(lx, 8, 12)]
+    np.testing.assert_array_equal(views[0][:, :, :8], views[1][:, :, :8])
+    np.testing.assert_array_equal(views[2][:, :, :8], views[3][:, :, :8])
+    assert np.abs(views[0][:, :, :8]).max() > 0
+    wide.run(1)
+    trunc.run(1)
+    tr = learned.Trainer(job_t.mx, job_t.mzp, max_groups=2, batch=2, residual=True)
+    for g in range(2):
+        for lev in range(4):
+            tr.capture(wide, g, lev)
+        tr.capture_residual(wide, trunc, g)
+    scales = tr.compute_scales(2)
+    tr.set_weights(np.zeros((3, learned.NSIDE), np.float32))
+    diff = (learned.band_host(_host_fields(wide), job_w.mx, job_w.mzp, job_w.tpml)
+            - learned.band_host(_host_fields(trunc), job_t.mx, job_t.mzp, job_t.tpml))
+    offs, lens = [0, lz * 96, 2 * lz * 96], [lz, lz, lx]
+    for s in range(3):
+        d = diff[offs[s]:offs[s] + lens[s] * 96].reshape(lens[s], 8, 12)[:, :, :8] / scales[s][None, :, None]
+        want = float((d.astype(np.float64) ** 2).mean())
+        got = tr.step(s, np.array([0, 1], np.int32), 0.0)
+        assert want > 0 and abs(got - want) <= 1e-5 * want, (s, got, want)
