@@ -182,6 +182,7 @@ def test_strip_copy_and_residual_capture():
     job_w = jobs.tiny_layered(steps=30, tpml=14)
     job_t = jobs.tiny_layered(steps=30, tpml=8)
     wide, trunc = lib.Simulation(job_w), lib.Simulation(job_t)
+    _random_state(wide, 11)   # every strip cell active from the start
     wide.run(6)
     trunc.run(6)
     learned.strip_copy(trunc, wide)
