@@ -375,8 +375,15 @@ def train(trainer: Trainer, weights: np.ndarray, iters: int, lr: float, val_frac
     perm = rng.permutation(n)
     nval = max(trainer.batch, int(n * val_frac))
     val, tr = perm[:nval], perm[nval:]
+    # the validation loss of a network that predicts nothing (zero output):
+    # the variance the training has to explain
+    trainer.set_weights(np.zeros_like(weights))
+    base = [float(np.mean([trainer.step(side, val[k:k + trainer.batch], 0.0)
+                           for k in range(0, len(val) - trainer.batch + 1, trainer.batch)]))
+            for side in range(3)]
+    log("[lb] zero-output validation loss: " + ", ".join(f"{SIDES[s]} {base[s]:.4e}" for s in range(3)))
     trainer.set_weights(weights)
-    hist = {"train": [[] for _ in range(3)], "val": [[] for _ in range(3)]}
+    hist = {"train": [[] for _ in range(3)], "val": [[] for _ in range(3)], "zero_val": base}
     for side in range(3):
         for it in range(iters):
             a = lr * 0.5 * (1.0 + np.cos(np.pi * it / iters))

@@ -104,6 +104,7 @@ def main():
                "iters": args.iters, "lr": args.lr,
                "final_train_loss": [h[-1] for h in hist["train"]],
                "final_val_loss": [h[-1] for h in hist["val"]],
+               "zero_output_val_loss": hist["zero_val"],
                "check": res}
     net.save(args.out, train_loss=np.array(hist["train"]), val_loss=np.array(hist["val"]),
              info=json.dumps(summary))
