@@ -51,7 +51,7 @@ def _lib():
         sig = {
             "bq_band_size": ([C.c_int, C.c_int], C.c_int),
             "bq_band_extract": ([p, p], C.c_int),
-            "bq_lb_create": ([p, p, C.c_int, C.c_float, C.POINTER(p)], C.c_int),
+            "bq_lb_create": ([p, p, C.c_int, C.c_float, C.c_int, C.POINTER(p)], C.c_int),
             "bq_strip_copy": ([p, p], C.c_int),
             "bq_lb_trainer_capture_residual": ([p, p, p, C.c_int], C.c_int),
             "bq_lb_destroy": ([p], None),
@@ -60,8 +60,8 @@ def _lib():
             "bq_lb_detach": ([p], C.c_int),
             "bq_lb_infer": ([p, C.c_int, C.c_int, p, p], C.c_int),
             "bq_field_misfit": ([p, p, C.c_int, p], C.c_int),
-            "bq_lb_trainer_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(p)],
-                                     C.c_int),
+            "bq_lb_trainer_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(p)], C.c_int),
             "bq_lb_trainer_destroy": ([p], None),
             "bq_lb_trainer_last_error": ([p], C.c_char_p),
             "bq_lb_trainer_capture": ([p, p, C.c_int, C.c_int], C.c_int),
@@ -141,33 +141,36 @@ class Network:
     gain; on the device as a ``bq_lb`` for :meth:`attach`."""
 
     def __init__(self, weights: np.ndarray, scales: np.ndarray, residual: bool = True,
-                 gain: float = 1.0):
+                 gain: float = 1.0, inner: int = S):
         self.weights = np.ascontiguousarray(weights, np.float32).reshape(3, NSIDE)
         self.scales = np.ascontiguousarray(scales, np.float32).reshape(3, F)
         self.residual = bool(residual)
         self.gain = float(gain)
+        self.inner = int(inner)
         self._h = None
 
     @classmethod
     def load(cls, path: str = WEIGHTS, gain: float | None = None) -> "Network":
         z = np.load(path, allow_pickle=False)
         g = float(z["gain"]) if gain is None and "gain" in z else (1.0 if gain is None else gain)
-        return cls(z["weights"], z["scales"], bool(z["residual"]) if "residual" in z else True, g)
+        return cls(z["weights"], z["scales"], bool(z["residual"]) if "residual" in z else True, g,
+                   int(z["inner"]) if "inner" in z else S)
 
     def with_gain(self, gain: float) -> "Network":
-        return Network(self.weights, self.scales, self.residual, gain)
+        return Network(self.weights, self.scales, self.residual, gain, self.inner)
 
     def save(self, path: str, **info):
         os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
         np.savez_compressed(path, weights=self.weights, scales=self.scales,
                             residual=np.asarray(self.residual), gain=np.asarray(self.gain),
+                            inner=np.asarray(self.inner),
                             **{k: np.asarray(v) for k, v in info.items()})
 
     def device(self):
         if self._h is None:
             h = C.c_void_p()
             rc = _lib().bq_lb_create(_ptr(self.weights), _ptr(self.scales), int(self.residual),
-                                     float(self.gain), C.byref(h))
+                                     float(self.gain), self.inner, C.byref(h))
             if rc:
                 raise BiotError(f"bq_lb_create failed ({rc}): {_lib().bq_lb_last_error().decode()}")
             self._h = h
@@ -236,14 +239,17 @@ def field_misfit(a: Simulation, b: Simulation, field: str) -> np.ndarray:
 class Trainer:
     """Band store and FP32 training kernels on the device (bq_lb_trainer)."""
 
-    def __init__(self, mx: int, mzp: int, max_groups: int, batch: int = 8, residual: bool = True):
+    def __init__(self, mx: int, mzp: int, max_groups: int, batch: int = 8, residual: bool = True,
+                 inner: int = S):
         h = C.c_void_p()
-        rc = _lib().bq_lb_trainer_create(mx, mzp, max_groups, batch, int(residual), C.byref(h))
+        rc = _lib().bq_lb_trainer_create(mx, mzp, max_groups, batch, int(residual), int(inner),
+                                         C.byref(h))
         if rc:
             raise BiotError(f"bq_lb_trainer_create failed ({rc}): "
                             f"{_lib().bq_lb_trainer_last_error(None).decode()}")
         self.t, self.mx, self.mzp, self.max_groups, self.batch = h, mx, mzp, max_groups, batch
         self.residual = bool(residual)
+        self.inner = int(inner)
         self.ngroups = 0
         self.scales = np.ones((3, F), np.float32)
 

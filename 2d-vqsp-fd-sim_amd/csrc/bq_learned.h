@@ -19,7 +19,9 @@
 //
 // In the direct form the network is the strip's whole update; in the residual
 // form it corrects the update the 8-cell CPML has just made, and gain 0 is
-// the plain CPML.
+// the plain CPML. Only the `inner` strip cells nearest the interior receive
+// the output; the others keep their CPML update (residual) or their previous
+// level's values plus the change (direct, inner = 8).
 //
 // conv5 is a 5-tap convolution along the boundary with zero padding at the
 // ends of each layer's input, no bias; lrelu(y) = max(y, 0.1 y). Inference
@@ -58,6 +60,7 @@ struct LbNet {
   float scale[3][kLbF], inv[3][kLbF];
   int residual;   // 0: direct form, 1: residual form (see above)
   float gain;
+  int inner;      // strip cells the output is applied to, counted from the interior (1..8)
 };
 
 __host__ __device__ inline int lb_lz(int MZ) { return MZ + 1; }
@@ -193,7 +196,7 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
     for (int j = 0; j < 4; ++j) {
       const int r = 16 * v + 4 * (lane >> 4) + j, co = 16 * n + (lane & 15);
       const int p = p0 + r;
-      if (r < kLbP && p < L) {
+      if (r < kLbP && p < L && (co & 7) >= kLbS - net.inner) {
         const int f = co >> 3, c = co & 7;
         const float prev = lev[3][soff + (size_t)p * kLbBand + f * kLbCells + c];
         put(p, f, c, prev, net.gain * net.scale[side][f] * acc[n][j]);

@@ -159,14 +159,16 @@ __global__ __launch_bounds__(kTB) void k_tr_fwd(const float *__restrict__ x, con
   a[e] = act ? (s > 0.f ? s : 0.1f * s) : s;
 }
 
-// dout = 2 (out - y) / n; part[block] = sum (out - y)^2 of the block.
+// dout = 2 (out - y) / n; part[block] = sum (out - y)^2 of the block. Only
+// the output channels of the `inner` strip cells nearest the interior count.
 __global__ __launch_bounds__(kTB) void k_tr_loss(const float *__restrict__ out, const float *__restrict__ y,
-                                                 float *__restrict__ dout, size_t n, double *__restrict__ part) {
+                                                 float *__restrict__ dout, size_t n, int inner,
+                                                 double *__restrict__ part) {
   __shared__ double sh[kTB];
   const size_t e = (size_t)blockIdx.x * kTB + threadIdx.x;
   double d2 = 0.0;
   if (e < n) {
-    const float d = out[e] - y[e];
+    const float d = (int)(e & 7) >= kLbS - inner ? out[e] - y[e] : 0.f;
     dout[e] = 2.0f * d / (float)n;
     d2 = (double)d * d;
   }
@@ -336,6 +338,7 @@ struct bq_lb_trainer {
   int nparts = 0;
   int adam_t[3] = {0, 0, 0};
   int residual = 0;
+  int inner = kLbS;
   float scale[3][kLbF];
   std::string err;
 };
@@ -379,8 +382,9 @@ int grid_of(size_t n) { return (int)std::min<size_t>((n + kTB - 1) / kTB, 1 << 2
 
 extern "C" {
 
-int bq_lb_create(const float *weights, const float *scales, int residual, float gain, bq_lb **out) {
-  if (!out || !weights || !scales) return -1;
+int bq_lb_create(const float *weights, const float *scales, int residual, float gain, int inner,
+                 bq_lb **out) {
+  if (!out || !weights || !scales || inner < 1 || inner > kLbS) return -1;
   if (!(gain >= 0.f && gain <= 16.f)) {
     g_lb_err = "bq_lb_create: gain outside [0, 16]";
     return -2;
@@ -412,6 +416,7 @@ int bq_lb_create(const float *weights, const float *scales, int residual, float 
   }
   lb->net.residual = residual != 0;
   lb->net.gain = gain;
+  lb->net.inner = inner;
   for (int s = 0; s < 3; ++s) {
     const unsigned short *base = lb->w + (size_t)s * kNSide;
     lb->net.w1[s] = base;
@@ -552,12 +557,14 @@ int bq_field_misfit(bq_handle *a, bq_handle *b, int field_id, double out[2]) {
 
 // ---------------------------------------------------------------- trainer
 
-int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual,
+int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual, int inner,
                          bq_lb_trainer **out) {
-  if (!out || mx < 8 || mzp < 6 || max_groups < 1 || batch < 1) return -1;
+  if (!out || mx < 8 || mzp < 6 || max_groups < 1 || batch < 1 || inner < 1 || inner > kLbS)
+    return -1;
   *out = nullptr;
   bq_lb_trainer *t = new bq_lb_trainer();
   t->residual = residual != 0;
+  t->inner = inner;
   (void)hipGetDevice(&t->device);
   t->MX = mx;
   t->MZ = mzp;
@@ -699,7 +706,8 @@ int bq_lb_trainer_step(bq_lb_trainer *t, int side, const int32_t *groups, float 
   hipLaunchKernelGGL(k_tr_fwd, dim3(go), dim3(kTB), 0, st, t->x, w1, t->y1, t->a1, B, L, kLbCin, 1);
   hipLaunchKernelGGL(k_tr_fwd, dim3(go), dim3(kTB), 0, st, t->a1, w2, t->y2, t->a2, B, L, kLbH, 1);
   hipLaunchKernelGGL(k_tr_fwd, dim3(go), dim3(kTB), 0, st, t->a2, w3, t->y3, t->o3, B, L, kLbH, 0);
-  hipLaunchKernelGGL(k_tr_loss, dim3(go), dim3(kTB), 0, st, t->o3, t->y, t->d3, nout, t->part);
+  hipLaunchKernelGGL(k_tr_loss, dim3(go), dim3(kTB), 0, st, t->o3, t->y, t->d3, nout, t->inner,
+                     t->part);
   std::vector<double> h(go);
   if (hipMemcpyAsync(h.data(), t->part, go * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
     return tfail(t, -100, "loss copy failed");
@@ -730,7 +738,7 @@ int bq_lb_trainer_step(bq_lb_trainer *t, int side, const int32_t *groups, float 
   if (loss) {
     double s = 0.0;
     for (double v : h) s += v;
-    *loss = s / (double)nout;
+    *loss = s / ((double)nout * t->inner / kLbS);
   }
   return 0;
 }

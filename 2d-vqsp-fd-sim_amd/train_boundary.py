@@ -83,13 +83,15 @@ def main():
     ap.add_argument("--direct", action="store_true",
                     help="train the direct form (the network is the strip's whole update) "
                          "instead of the residual form (a correction of the 8-cell CPML)")
+    ap.add_argument("--inner", type=int, default=8,
+                    help="strip cells nearest the interior that the network corrects (1..8)")
     ap.add_argument("--gains", default="1,0.5,0.25",
                     help="output gains to check on the held-out models")
     args = ap.parse_args()
     t0 = time.perf_counter()
     per_model = len(range(args.first, args.steps - 1, args.every))
     tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch,
-                         residual=not args.direct)
+                         residual=not args.direct, inner=args.inner)
     ng = learned.record(tr, args.models, args.steps, args.first, args.every, log=log)
     t_rec = time.perf_counter() - t0
     scales = tr.compute_scales(ng)
@@ -97,11 +99,12 @@ def main():
     t1 = time.perf_counter()
     hist = learned.train(tr, learned.init_weights(args.seed), args.iters, args.lr, seed=args.seed, log=log)
     t_train = time.perf_counter() - t1
-    net = learned.Network(tr.get_weights(), scales, residual=not args.direct)
+    net = learned.Network(tr.get_weights(), scales, residual=not args.direct, inner=args.inner)
     gains = [float(g) for g in args.gains.split(",")]
     res = check(net, gains, args.check, args.steps, 500) if args.check else []
     summary = {"groups": ng, "record_s": round(t_rec, 1), "train_s": round(t_train, 1),
-               "iters": args.iters, "lr": args.lr,
+               "iters": args.iters, "lr": args.lr, "residual": not args.direct,
+               "inner": args.inner,
                "final_train_loss": [h[-1] for h in hist["train"]],
                "final_val_loss": [h[-1] for h in hist["val"]],
                "zero_output_val_loss": hist["zero_val"],

@@ -54,8 +54,11 @@ int bq_band_extract(bq_handle *h, float *dst);
 
 /* Networks of the three sides from FP32 weights and scales (layout above),
  * converted to BF16 on the device that is current; residual 0/1 selects the
- * form, gain (0..16) scales the output. */
-int bq_lb_create(const float *weights, const float *scales, int residual, float gain, bq_lb **out);
+ * form, gain (0..16) scales the output, inner (1..8) is the number of strip
+ * cells, counted from the interior, the output is applied to (the others
+ * keep their update). */
+int bq_lb_create(const float *weights, const float *scales, int residual, float gain, int inner,
+                 bq_lb **out);
 void bq_lb_destroy(bq_lb *lb);
 const char *bq_lb_last_error(void);
 
@@ -87,8 +90,9 @@ int bq_field_misfit(bq_handle *a, bq_handle *b, int field_id, double out[2]);
  * the band of an 8-cell grid kept on its trajectory (bq_strip_copy) after
  * that grid's own step t+1 (bq_lb_trainer_capture_residual), and the target
  * its strip cells. All runs recorded must have the trainer's MX, MZ' and
- * TPML >= 8. */
-int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual,
+ * TPML >= 8. inner (1..8): only the targets of the strip cells nearest the
+ * interior enter the loss. */
+int bq_lb_trainer_create(int mx, int mzp, int max_groups, int batch, int residual, int inner,
                          bq_lb_trainer **out);
 void bq_lb_trainer_destroy(bq_lb_trainer *t);
 const char *bq_lb_trainer_last_error(bq_lb_trainer *t);
@@ -103,7 +107,7 @@ int bq_lb_trainer_scales(bq_lb_trainer *t, int ngroups, float *scales);
 int bq_lb_trainer_set_weights(bq_lb_trainer *t, const float *weights);
 int bq_lb_trainer_get_weights(bq_lb_trainer *t, float *weights);
 /* One minibatch of `batch` groups (indices) of side `side`: mean squared error
- * of the scaled change into *loss; with lr > 0 also backward and an Adam step
+ * of the scaled target over the inner cells into *loss; with lr > 0 also backward and an Adam step
  * (beta 0.9, 0.999). */
 int bq_lb_trainer_step(bq_lb_trainer *t, int side, const int32_t *groups, float lr, double *loss);
 

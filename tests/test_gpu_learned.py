@@ -57,11 +57,11 @@ def _host_fields(sim):
     return {f: sim.field(f) for f in FIELDS}
 
 
-@pytest.mark.parametrize("residual", [False, True])
-def test_band_extract_matches_host_and_training_kernels_match_torch(residual):
+@pytest.mark.parametrize("residual,inner", [(False, 8), (True, 3)])
+def test_band_extract_matches_host_and_training_kernels_match_torch(residual, inner):
     import torch
     job = jobs.tiny_layered(steps=30, tpml=8)
-    tr = learned.Trainer(job.mx, job.mzp, max_groups=4, batch=2, residual=residual)
+    tr = learned.Trainer(job.mx, job.mzp, max_groups=4, batch=2, residual=residual, inner=inner)
     host = []   # [group][level] bands on the host
     for g in range(4):
         sim = lib.Simulation(job)
@@ -107,7 +107,8 @@ def test_band_extract_matches_host_and_training_kernels_match_torch(residual):
             h = torch.nn.functional.conv1d(h, w, padding=2)
             if k < 2:
                 h = torch.nn.functional.leaky_relu(h, 0.1)
-        loss = ((h - y) ** 2).mean()
+        mask = torch.tensor((np.arange(64) % 8) >= 8 - inner)[None, :, None].expand_as(h)
+        loss = ((h - y) ** 2)[mask].mean()
         loss.backward()
         got = tr.step(s, groups, 0.0)
         assert abs(got - loss.item()) <= 1e-4 * loss.item(), (s, got, loss.item())
@@ -129,8 +130,8 @@ def test_band_extract_matches_host_and_training_kernels_match_torch(residual):
     tr.close()
 
 
-@pytest.mark.parametrize("residual,gain", [(False, 1.0), (True, 0.5)])
-def test_attached_network_replaces_only_the_strip(residual, gain):
+@pytest.mark.parametrize("residual,gain,inner", [(False, 1.0, 8), (True, 0.5, 8), (True, 2.0, 2)])
+def test_attached_network_replaces_only_the_strip(residual, gain, inner):
     """One step from a random state: every cell outside the 8-cell strip is
     bit-identical to the plain 8-cell PML run; the strip cells are the
     network's prediction from the band of the start level (direct form), or
@@ -142,7 +143,7 @@ def test_attached_network_replaces_only_the_strip(residual, gain):
     band0 = learned.band_host(_host_fields(run), job.mx, job.mzp, job.tpml)
     w = _weights(3, last=10.0)
     scales = np.full((3, 8), 0.7, np.float32)
-    net = learned.Network(w, scales, residual=residual, gain=gain)
+    net = learned.Network(w, scales, residual=residual, gain=gain, inner=inner)
     net.attach(run)
     plain.run(1)
     run.run(1)
@@ -168,6 +169,8 @@ def test_attached_network_replaces_only_the_strip(residual, gain):
             strip[ll, ii - pmll] = True
             got = a[ll, ii - pmll]
             want = b[ll, ii - pmll] + (pred - prev) if residual else pred
+            outer = np.arange(8) < 8 - inner       # strip cells the network leaves alone
+            want[:, outer] = b[ll, ii - pmll][:, outer]
             scale = np.abs(pred - prev).max() + 1e-30
             assert np.abs(got - want).max() <= 2e-2 * scale + 1e-6 * np.abs(want).max(), (name, s)
         np.testing.assert_array_equal(a[~strip], b[~strip], err_msg=name)
