@@ -20,8 +20,7 @@
 // In the direct form the network is the strip's whole update; in the residual
 // form it corrects the update the 8-cell CPML has just made, and gain 0 is
 // the plain CPML. Only the `inner` strip cells nearest the interior receive
-// the output; the others keep their CPML update (residual) or their previous
-// level's values plus the change (direct, inner = 8).
+// the output; the others keep the value the 8-cell CPML computed.
 //
 // conv5 is a 5-tap convolution along the boundary with zero padding at the
 // ends of each layer's input, no bias; lrelu(y) = max(y, 0.1 y). Inference
