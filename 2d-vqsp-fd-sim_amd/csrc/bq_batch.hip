@@ -9,6 +9,9 @@
 //                previous step
 //   kb_velocity  the velocity half-step of every shot's other cells
 //   kb_edges     row 1, row PMLZH and the bottom history of every shot
+//   kb_lb_apply, kb_lb_extract   for the shots with a learned boundary
+//                attached: the strip cells from the network, then the band
+//                of the new level into the shot's ring
 //
 // A block finds its shot from the prefix sums of the shots' block counts and
 // runs exactly the per-shot device code of the single-grid launches, so every
@@ -29,6 +32,7 @@
 
 #include "../../include/biotfd.h"
 #include "bq_handle.h"
+#include "bq_learned.h"
 
 namespace {
 
@@ -45,10 +49,14 @@ struct ShotDev {
   int vrest, vruns;                    // kb_velocity blocks
   int nedge;                           // kb_edges blocks
   int base_step, base_par;             // step number and parity at the run's start
+  int lb_on;                           // learned boundary attached
+  LbNet lb;
+  float *ring;                         // its band levels, level n in slot n % 4
+  int nlb, nlbx;                       // kb_lb_apply, kb_lb_extract blocks
 };
 
 // Prefix sums over the shots of each launch's blocks ([nshots + 1] each).
-enum { C_FUSED = 0, C_OTHER, C_VEL, C_EDGE, C_SAMPLE, C_N };
+enum { C_FUSED = 0, C_OTHER, C_VEL, C_EDGE, C_SAMPLE, C_LB, C_LBX, C_N };
 
 struct BatchArgs {
   const ShotDev *shots;
@@ -213,6 +221,51 @@ __global__ __launch_bounds__(kEdgeBlock) void kb_edges(BatchArgs B, int kl) {
   edges_block(k, blockIdx.x - B.cum[C_EDGE * (B.nshots + 1) + s], su, sq);
 }
 
+// Learned boundary of the shots that have one, after the step (the step's
+// parameters: its "next" set holds the new level). Block b: a tile of one
+// side of one shot.
+__global__ __launch_bounds__(kLbBlock) void kb_lb_apply(BatchArgs B, int kl) {
+  __shared__ LbLds S;
+  const int kofs = *B.dstep + kl;
+  const int s = find_shot(B.cum + C_LB * (B.nshots + 1), B.nshots, blockIdx.x);
+  const ShotDev &sd = B.shots[s];
+  const KParams &k = sd.kp[(sd.base_par + kofs) & 1];
+  const int it = sd.base_step + kofs;
+  const int MX = k.MX, MZ = k.MZ, P = k.P, OFF = k.OFF;
+  float *const U = k.U1, *const W = k.W1, *const QX = k.QX1, *const QZ = k.QZ1;
+  float *const TXX = k.TXX1, *const TZZ = k.TZZ1, *const TXZ = k.TXZ1, *const PR = k.PR1;
+  const int residual = sd.lb.residual;
+  const size_t bs = lb_band_size(MX, MZ);
+  const LbLevels lev{sd.ring + (size_t)((it + 0) & 3) * bs, sd.ring + (size_t)((it + 1) & 3) * bs,
+                     sd.ring + (size_t)((it + 2) & 3) * bs, sd.ring + (size_t)((it + 3) & 3) * bs};
+  const int b = blockIdx.x - B.cum[C_LB * (B.nshots + 1) + s];
+  const int t0 = lb_tiles(lb_lz(MZ)), t1 = 2 * t0;
+  const int side = b < t0 ? 0 : (b < t1 ? 1 : 2);
+  const int tile = b - (side == 0 ? 0 : side == 1 ? t0 : t1);
+  lb_tile(sd.lb, side, lb_side_len(side, MX, MZ), lb_side_off(side, MZ), tile * kLbP, lev, S,
+          [=](int p, int f, int c, float prev, float d) {
+            int i, l;
+            lb_cell(side, p, c, MX, MZ, &i, &l);
+            float *F = f == 0 ? U : f == 1 ? W : f == 2 ? QX : f == 3 ? QZ
+                     : f == 4 ? TXX : f == 5 ? TZZ : f == 6 ? TXZ : PR;
+            float &x = F[(size_t)l * P + (size_t)(i + OFF)];
+            x = (residual ? x : prev) + d;
+          });
+}
+
+__global__ __launch_bounds__(kBlock) void kb_lb_extract(BatchArgs B, int kl) {
+  const int kofs = *B.dstep + kl;
+  const int s = find_shot(B.cum + C_LBX * (B.nshots + 1), B.nshots, blockIdx.x);
+  const ShotDev &sd = B.shots[s];
+  const KParams k = sd.kp[(sd.base_par + kofs) & 1];
+  const int it = sd.base_step + kofs;
+  const size_t bs = lb_band_size(k.MX, k.MZ);
+  const size_t e = (size_t)(blockIdx.x - B.cum[C_LBX * (B.nshots + 1) + s]) * kBlock + threadIdx.x;
+  if (e >= bs) return;
+  const float *F[8] = {k.U1, k.W1, k.QX1, k.QZ1, k.TXX1, k.TZZ1, k.TXZ1, k.PR1};
+  lb_extract(sd.ring + (size_t)(it & 3) * bs, F, k.P, k.OFF, k.MX, k.MZ, e, 0);
+}
+
 // Adds n steps to the run's step counter, after every launch that reads it.
 __global__ void kb_advance(int *dstep, int n) {
   if (threadIdx.x == 0) *dstep += n;
@@ -245,14 +298,14 @@ struct bq_batch {
   ShotDev *dshots = nullptr;
   int *dcum = nullptr;     // [C_N][nshots + 1]
   int *dctl = nullptr;     // [0]: step counter, [8..19]: persistent queue
-  int total[C_N] = {0, 0, 0, 0, 0};
+  int total[C_N] = {0, 0, 0, 0, 0, 0, 0};
   int fr = 128;            // fused tile height
   int sched = BQ_SCHED_GRID;
   int use_graph = 1;
   int cus = 256;
   hipGraphExec_t graph = nullptr;
   int graph_fr = 0, graph_sched = -1;
-  int graph_total[C_N] = {-1, -1, -1, -1, -1};   // block counts the graph was made for
+  int graph_total[C_N] = {-1, -1, -1, -1, -1, -1, -1};   // block counts the graph was made for
   hipEvent_t ev[2] = {nullptr, nullptr};
   int last_steps = 0;
   std::vector<double> tune_ms;   // per candidate, ms per step
@@ -305,8 +358,11 @@ int layout(bq_batch *b, int fr) {
     d.vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;
     d.vruns = h->sv.qcum[kMaxRuns];
     d.nedge = (h->g.NX + kEdgeBlock - 1) / kEdgeBlock;
+    d.lb_on = h->lb != nullptr;
+    d.nlb = d.lb_on ? 2 * lb_tiles(lb_lz(h->g.MZ)) + lb_tiles(lb_lx(h->g.MX)) : 0;
+    d.nlbx = d.lb_on ? (int)((lb_band_size(h->g.MX, h->g.MZ) + kBlock - 1) / kBlock) : 0;
     const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample, d.vrest + d.vruns, d.nedge,
-                          d.nsample};
+                          d.nsample, d.nlb, d.nlbx};
     for (int c = 0; c < C_N; ++c) cum[(size_t)c * (n + 1) + s + 1] = cum[(size_t)c * (n + 1) + s] + cnt[c];
   }
   for (int c = 0; c < C_N; ++c) b->total[c] = cum[(size_t)c * (n + 1) + n];
@@ -326,6 +382,8 @@ int upload_shots(bq_batch *b) {
     d.kp[1] = params_for(h, 1);
     d.base_step = h->next_step;
     d.base_par = h->parity;
+    d.ring = h->lb_ring;
+    if (h->lb) d.lb = bq::lb_net_of(h->lb);
   }
   BCHK(b, hipMemcpyAsync(b->dshots, b->host.data(), b->host.size() * sizeof(ShotDev),
                          hipMemcpyHostToDevice, b->stream));
@@ -351,6 +409,10 @@ void launch_step_fr(bq_batch *b, const BatchArgs &a, int kl, int advance) {
     if (nvel) hipLaunchKernelGGL((kb_velocity<false>), dim3(nvel), dim3(kBlock), 0, b->stream, a, kl);
   }
   hipLaunchKernelGGL(kb_edges, dim3(a.total[C_EDGE]), dim3(kEdgeBlock), 0, b->stream, a, kl);
+  if (a.total[C_LB]) {
+    hipLaunchKernelGGL(kb_lb_apply, dim3(a.total[C_LB]), dim3(kLbBlock), 0, b->stream, a, kl);
+    hipLaunchKernelGGL(kb_lb_extract, dim3(a.total[C_LBX]), dim3(kBlock), 0, b->stream, a, kl);
+  }
   if (advance) hipLaunchKernelGGL(kb_advance, dim3(1), dim3(64), 0, b->stream, a.dstep, advance);
 }
 

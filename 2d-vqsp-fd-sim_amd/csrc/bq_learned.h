@@ -114,6 +114,15 @@ __device__ __forceinline__ unsigned short lb_bf16(float x) {
 typedef short lb_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float lb_f32x4 __attribute__((ext_vector_type(4)));
 
+// The band at the last four time levels, oldest first (level t = p3).
+struct LbLevels {
+  const float *p0, *p1, *p2, *p3;
+};
+
+__device__ __forceinline__ const float *lb_level(const LbLevels &v, int lv) {
+  return lv == 0 ? v.p0 : lv == 1 ? v.p1 : lv == 2 ? v.p2 : v.p3;
+}
+
 struct LbLds {
   unsigned short x[kLbRows * kLbXS];
   unsigned short h1[kLbRows * kLbHS];
@@ -164,13 +173,13 @@ __device__ __forceinline__ void lb_store_hidden(unsigned short *h, const lb_f32x
   for (int e = threadIdx.x; e < 4 * kLbHS; e += kLbBlock) h[64 * kLbHS + e] = 0;
 }
 
-// One tile of one side: positions p0 .. p0+51. lev[0..3]: the band at the
-// last four time levels, oldest first (level t = lev[3]). For each strip
+// One tile of one side: positions p0 .. p0+51. lev: the band at the last
+// four time levels, oldest first. For each strip
 // cell put(p, f, c, prev, d) receives lev[3]'s value and the network's
 // gain * scale * output.
 template <class Put>
 __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_t soff, int p0,
-                                        const float *const *lev, LbLds &S, Put put) {
+                                        const LbLevels lev, LbLds &S, Put put) {
   const float *inv = net.inv[side];
   // input rows r at positions p0 - 6 + r
   for (int e = threadIdx.x; e < kLbRows * kLbCin; e += kLbBlock) {
@@ -178,7 +187,7 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
     const int p = p0 - 6 + r;
     const int lv = ci / kLbBand, fc = ci % kLbBand;
     float x = 0.f;
-    if (p >= 0 && p < L) x = lev[lv][soff + (size_t)p * kLbBand + fc] * inv[fc / kLbCells];
+    if (p >= 0 && p < L) x = lb_level(lev, lv)[soff + (size_t)p * kLbBand + fc] * inv[fc / kLbCells];
     S.x[r * kLbXS + ci] = lb_bf16(x);
   }
   __syncthreads();
@@ -197,7 +206,7 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
       const int p = p0 + r;
       if (r < kLbP && p < L && (co & 7) >= kLbS - net.inner) {
         const int f = co >> 3, c = co & 7;
-        const float prev = lev[3][soff + (size_t)p * kLbBand + f * kLbCells + c];
+        const float prev = lev.p3[soff + (size_t)p * kLbBand + f * kLbCells + c];
         put(p, f, c, prev, net.gain * net.scale[side][f] * acc[n][j]);
       }
     }
@@ -206,3 +215,8 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
 __host__ __device__ inline int lb_tiles(int L) { return (L + kLbP - 1) / kLbP; }
 
 }  // namespace bqk
+
+namespace bq {
+// The networks of a bq_lb (bq_learned.hip).
+const bqk::LbNet &lb_net_of(const void *lb);
+}  // namespace bq

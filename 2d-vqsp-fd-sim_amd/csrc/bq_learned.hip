@@ -67,14 +67,10 @@ __global__ __launch_bounds__(kTB) void k_strip_copy(F8 A, int pa, int offa, F8 B
   }
 }
 
-// lev[j] = the band at level t-3+j
-struct Lev4 {
-  const float *p[4];
-};
 
 // After a step: strip cells of all three sides from the network; block b is a
 // tile of 52 positions of one side.
-__global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, Lev4 lev) {
+__global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, LbLevels lev) {
   __shared__ LbLds S;
   const int MX = k.MX, MZ = k.MZ;
   const int t0 = lb_tiles(lb_lz(MZ)), t1 = 2 * t0;
@@ -82,7 +78,7 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, Lev
   const int side = b < t0 ? 0 : (b < t1 ? 1 : 2);
   const int tile = b - (side == 0 ? 0 : side == 1 ? t0 : t1);
   const int L = lb_side_len(side, MX, MZ);
-  lb_tile(net, side, L, lb_side_off(side, MZ), tile * kLbP, lev.p, S,
+  lb_tile(net, side, L, lb_side_off(side, MZ), tile * kLbP, lev, S,
           [&](int p, int f, int c, float prev, float d) {
             int i, l;
             lb_cell(side, p, c, MX, MZ, &i, &l);
@@ -96,10 +92,10 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, Lev
 
 // The network on given bands of one side (length L, levels [4][L][96]):
 // out[p][f][c] = the new strip values.
-__global__ __launch_bounds__(kLbBlock) void k_lb_infer(LbNet net, int side, int L, Lev4 lev,
+__global__ __launch_bounds__(kLbBlock) void k_lb_infer(LbNet net, int side, int L, LbLevels lev,
                                                        float *__restrict__ out) {
   __shared__ LbLds S;
-  lb_tile(net, side, L, 0, blockIdx.x * kLbP, lev.p, S, [&](int p, int f, int c, float prev, float d) {
+  lb_tile(net, side, L, 0, blockIdx.x * kLbP, lev, S, [&](int p, int f, int c, float prev, float d) {
     out[((size_t)p * kLbF + f) * kLbS + c] = prev + d;
   });
 }
@@ -344,14 +340,16 @@ struct bq_lb_trainer {
 };
 
 namespace bq {
+const LbNet &lb_net_of(const void *lb) { return ((const bq_lb *)lb)->net; }
+
 // After step `it` of a grid with a network attached: strip cells from the
 // network (levels it-4 .. it-1 in the ring), then the band of level it into
 // the ring. k: the step's parameters (the step wrote the "next" set).
 int lb_after_step(bq_handle *h, const KParams &k, int it) {
   const bq_lb *lb = (const bq_lb *)h->lb;
   const size_t bs = lb_band_size(k.MX, k.MZ);
-  Lev4 lev;
-  for (int j = 0; j < 4; ++j) lev.p[j] = h->lb_ring + (size_t)((it + j) & 3) * bs;
+  const LbLevels lev{h->lb_ring + (size_t)((it + 0) & 3) * bs, h->lb_ring + (size_t)((it + 1) & 3) * bs,
+                     h->lb_ring + (size_t)((it + 2) & 3) * bs, h->lb_ring + (size_t)((it + 3) & 3) * bs};
   const int tiles = 2 * lb_tiles(lb_lz(k.MZ)) + lb_tiles(lb_lx(k.MX));
   hipLaunchKernelGGL(k_lb_apply, dim3(tiles), dim3(kLbBlock), 0, h->stream, lb->net, k, lev);
   const int blocks = (int)std::min<size_t>((bs + kTB - 1) / kTB, 4096);
@@ -517,8 +515,7 @@ int bq_lb_infer(bq_lb *lb, int side, int len, const float *bands, float *out) {
   }
   int rc = hipMemcpy(d, bands, 4 * n * 4, hipMemcpyHostToDevice) == hipSuccess ? 0 : -100;
   if (!rc) {
-    Lev4 lev;
-    for (int j = 0; j < 4; ++j) lev.p[j] = d + j * n;
+    const LbLevels lev{d, d + n, d + 2 * n, d + 3 * n};
     hipLaunchKernelGGL(k_lb_infer, dim3(lb_tiles(len)), dim3(kLbBlock), 0, 0, lb->net, side, len, lev, o);
     if (hipDeviceSynchronize() != hipSuccess) rc = -100;
   }

@@ -126,3 +126,38 @@ def test_trace_misfit_on_device_matches_host():
     assert lib.trace_misfit(a, a)[0] == 0.0
     with pytest.raises(lib.BiotError, match="steps not run"):
         lib.trace_misfit(a, b, 1, STEPS + 1)
+
+
+def test_batch_with_learned_boundary_matches_shots_run_alone():
+    """Shots with a learned boundary attached (8-cell strip) in a batch with a
+    plain shot: bit-identical to each shot run alone, the strip update
+    included."""
+    from biotfd import learned
+    w = learned.init_weights(5)
+    for s in range(3):
+        w[s, learned.NW1 + learned.NW2:] *= 3.0
+    nets = [learned.Network(w, np.full((3, 8), 0.8, np.float32), residual=False),
+            learned.Network(w, np.full((3, 8), 1.3, np.float32), residual=True, gain=0.5, inner=3)]
+    js = [jobs.tiny_layered(steps=STEPS, tpml=8), jobs.medium_layered(steps=STEPS, tpml=8),
+          jobs.tiny_edges(steps=STEPS)]
+
+    def make():
+        sims = [lib.Simulation(j) for j in js]
+        for k, s in enumerate(sims[:2]):
+            rng = np.random.default_rng(40 + k)
+            for f in FIELDS:
+                s.set_field(f, (0.01 * rng.standard_normal((s.nz, s.nx))).astype(np.float32))
+            nets[k].attach(s)
+        return sims
+
+    batched, alone = make(), make()
+    b = lib.Batch(batched)
+    b.set_schedule(128, lib.SCHED_GRID, True)
+    b.run(1)
+    b.run(20)    # ten graph replays, one step without
+    for a in alone:
+        a.run(21)
+    for k, (s, a) in enumerate(zip(batched, alone)):
+        _compare(s, a, f"learned-boundary shot {k}")
+    assert np.isfinite(batched[0].field("U")).all()
+    b.close()
