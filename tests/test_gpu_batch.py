@@ -136,8 +136,9 @@ def test_batch_with_learned_boundary_matches_shots_run_alone():
     w = learned.init_weights(5)
     for s in range(3):
         w[s, learned.NW1 + learned.NW2:] *= 3.0
-    nets = [learned.Network(w, np.full((3, 8), 0.8, np.float32), residual=False),
-            learned.Network(w, np.full((3, 8), 1.3, np.float32), residual=True, gain=0.5, inner=3)]
+    # small gains keep these untrained networks' runs finite over the steps
+    nets = [learned.Network(w, np.full((3, 8), 0.8, np.float32), residual=False, gain=0.02),
+            learned.Network(w, np.full((3, 8), 1.3, np.float32), residual=True, gain=0.05, inner=3)]
     js = [jobs.tiny_layered(steps=STEPS, tpml=8), jobs.medium_layered(steps=STEPS, tpml=8),
           jobs.tiny_edges(steps=STEPS)]
 
@@ -159,5 +160,6 @@ def test_batch_with_learned_boundary_matches_shots_run_alone():
         a.run(21)
     for k, (s, a) in enumerate(zip(batched, alone)):
         _compare(s, a, f"learned-boundary shot {k}")
-    assert np.isfinite(batched[0].field("U")).all()
+    for s in batched:
+        assert np.isfinite(s.field("U")).all() and np.abs(s.field("U")).max() > 0
     b.close()
