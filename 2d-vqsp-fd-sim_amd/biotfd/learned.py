@@ -137,39 +137,49 @@ def _inputs(bands, scale_side):
 
 class Network:
     """Weights [3][NSIDE] and scales [3][8] of the three sides, the form
-    (residual: correcting the 8-cell CPML's update, or direct) and the output
-    gain; on the device as a ``bq_lb`` for :meth:`attach`."""
+    (residual: correcting the 8-cell CPML's update; damping: the same
+    correction limited to pulling each value toward zero; or direct) and the
+    output gain; on the device as a ``bq_lb`` for :meth:`attach`."""
 
     def __init__(self, weights: np.ndarray, scales: np.ndarray, residual: bool = True,
-                 gain: float = 1.0, inner: int = S):
+                 gain: float = 1.0, inner: int = S, damping: bool = False):
         self.weights = np.ascontiguousarray(weights, np.float32).reshape(3, NSIDE)
         self.scales = np.ascontiguousarray(scales, np.float32).reshape(3, F)
         self.residual = bool(residual)
+        self.damping = bool(damping) and self.residual
         self.gain = float(gain)
         self.inner = int(inner)
         self._h = None
+
+    @property
+    def form(self) -> int:
+        """0 direct, 1 residual, 2 damping (bq_lb_create's form)."""
+        return 2 if self.damping else int(self.residual)
 
     @classmethod
     def load(cls, path: str = WEIGHTS, gain: float | None = None) -> "Network":
         z = np.load(path, allow_pickle=False)
         g = float(z["gain"]) if gain is None and "gain" in z else (1.0 if gain is None else gain)
         return cls(z["weights"], z["scales"], bool(z["residual"]) if "residual" in z else True, g,
-                   int(z["inner"]) if "inner" in z else S)
+                   int(z["inner"]) if "inner" in z else S,
+                   bool(z["damping"]) if "damping" in z else False)
 
-    def with_gain(self, gain: float) -> "Network":
-        return Network(self.weights, self.scales, self.residual, gain, self.inner)
+    def with_gain(self, gain: float, damping: bool | None = None) -> "Network":
+        return Network(self.weights, self.scales, self.residual, gain, self.inner,
+                       self.damping if damping is None else damping)
 
     def save(self, path: str, **info):
         os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
         np.savez_compressed(path, weights=self.weights, scales=self.scales,
-                            residual=np.asarray(self.residual), gain=np.asarray(self.gain),
+                            residual=np.asarray(self.residual), damping=np.asarray(self.damping),
+                            gain=np.asarray(self.gain),
                             inner=np.asarray(self.inner),
                             **{k: np.asarray(v) for k, v in info.items()})
 
     def device(self):
         if self._h is None:
             h = C.c_void_p()
-            rc = _lib().bq_lb_create(_ptr(self.weights), _ptr(self.scales), int(self.residual),
+            rc = _lib().bq_lb_create(_ptr(self.weights), _ptr(self.scales), self.form,
                                      float(self.gain), self.inner, C.byref(h))
             if rc:
                 raise BiotError(f"bq_lb_create failed ({rc}): {_lib().bq_lb_last_error().decode()}")

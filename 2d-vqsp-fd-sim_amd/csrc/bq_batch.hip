@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kLbBlock) void kb_lb_apply(BatchArgs B, int kl) {
   const int MX = k.MX, MZ = k.MZ, P = k.P, OFF = k.OFF;
   float *const U = k.U1, *const W = k.W1, *const QX = k.QX1, *const QZ = k.QZ1;
   float *const TXX = k.TXX1, *const TZZ = k.TZZ1, *const TXZ = k.TXZ1, *const PR = k.PR1;
-  const int residual = sd.lb.residual;
+  const int form = sd.lb.form;
   const size_t bs = lb_band_size(MX, MZ);
   const LbLevels lev{sd.ring + (size_t)((it + 0) & 3) * bs, sd.ring + (size_t)((it + 1) & 3) * bs,
                      sd.ring + (size_t)((it + 2) & 3) * bs, sd.ring + (size_t)((it + 3) & 3) * bs};
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kLbBlock) void kb_lb_apply(BatchArgs B, int kl) {
             float *F = f == 0 ? U : f == 1 ? W : f == 2 ? QX : f == 3 ? QZ
                      : f == 4 ? TXX : f == 5 ? TZZ : f == 6 ? TXZ : PR;
             float &x = F[(size_t)l * P + (size_t)(i + OFF)];
-            x = (residual ? x : prev) + d;
+            x = lb_update(form, x, prev, d);
           });
 }
 

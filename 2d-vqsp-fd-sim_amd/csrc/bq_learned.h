@@ -16,11 +16,17 @@
 //   layer3 d  =       conv5(h2, W3:  64 -> 64)     d[p][field*8 + cell]
 //   strip(t+1) = strip(t) + gain * scale[field] * d            (direct)
 //   strip(t+1) = cpml(t+1) + gain * scale[field] * d           (residual)
+//   strip(t+1) = clamp(cpml(t+1) + gain * scale[field] * d,
+//                      between 0 and cpml(t+1))                (damping)
 //
 // In the direct form the network is the strip's whole update; in the residual
 // form it corrects the update the 8-cell CPML has just made, and gain 0 is
-// the plain CPML. Only the `inner` strip cells nearest the interior receive
-// the output; the others keep the value the 8-cell CPML computed.
+// the plain CPML. The damping form is the residual form limited so that the
+// network can only pull each strip value toward zero, never past it: a
+// sponge (a per-cell factor in [0, 1] after the CPML's update) whose factors
+// the network chooses, so it cannot feed energy back into the grid. Only the
+// `inner` strip cells nearest the interior receive the output; the others
+// keep the value the 8-cell CPML computed.
 //
 // conv5 is a 5-tap convolution along the boundary with zero padding at the
 // ends of each layer's input, no bias; lrelu(y) = max(y, 0.1 y). Inference
@@ -51,13 +57,14 @@ constexpr int kLbRows = 68;             // LDS rows of each activation tile
 constexpr int kLbXS = kLbCin + 8;       // LDS row stride of the input tile (bf16)
 constexpr int kLbHS = kLbH + 8;         // LDS row stride of the hidden tiles (bf16)
 constexpr int kLbBlock = 256;
+enum { kLbDirect = 0, kLbResidual = 1, kLbDamping = 2 };
 
 // Weights of the three networks in BF16, [tap][out][in] (in contiguous), and
 // the per-field scales of each side.
 struct LbNet {
   const unsigned short *w1[3], *w2[3], *w3[3];
   float scale[3][kLbF], inv[3][kLbF];
-  int residual;   // 0: direct form, 1: residual form (see above)
+  int form;       // kLbDirect, kLbResidual or kLbDamping (see above)
   float gain;
   int inner;      // strip cells the output is applied to, counted from the interior (1..8)
 };
@@ -123,7 +130,7 @@ __device__ __forceinline__ const float *lb_level(const LbLevels &v, int lv) {
   return lv == 0 ? v.p0 : lv == 1 ? v.p1 : lv == 2 ? v.p2 : v.p3;
 }
 
-struct LbLds {
+struct alignas(16) LbLds {
   unsigned short x[kLbRows * kLbXS];
   unsigned short h1[kLbRows * kLbHS];
   unsigned short h2[kLbRows * kLbHS];
@@ -131,41 +138,53 @@ struct LbLds {
 
 // One 5-tap convolution layer on the matrix cores for a tile of 64 output
 // rows: out[r][co] = sum_t sum_ci in[r + t][ci] * w[t][co][ci], in and out in
-// LDS (bf16, row strides is / os), w in global memory. Wave v computes rows
-// 16v..16v+15 for all 64 outputs. The MFMA 16x16x16 operand layout: lane q
-// holds A[row q % 16][k 4(q/16) .. +3] and B[k 4(q/16) .. +3][col q % 16];
-// the result holds D[row 4(q/16) + j][col q % 16], j = 0..3.
+// LDS (bf16, row stride is), w in global memory. Wave v computes the 16
+// outputs 16v..16v+15 for all 64 rows (four 16-row tiles), so each weight
+// fragment is loaded once per block and used four times. The MFMA 16x16x16
+// operand layout: lane q holds A[row q % 16][k 4(q/16) .. +3] and
+// B[k 4(q/16) .. +3][col q % 16]; the result holds D[row 4(q/16) + j][col
+// q % 16], j = 0..3.
 template <int CIN>
 __device__ __forceinline__ void lb_layer(const unsigned short *in, int is,
                                          const unsigned short *__restrict__ w, lb_f32x4 acc[4]) {
   const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
   const int qr = lane & 15, qk = 4 * (lane >> 4);
-  for (int n = 0; n < 4; ++n) acc[n] = lb_f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < 4; ++m) acc[m] = lb_f32x4{0.f, 0.f, 0.f, 0.f};
   for (int t = 0; t < kLbTap; ++t) {
-    const unsigned short *arow = in + (size_t)(16 * v + qr + t) * is + qk;
-    const unsigned short *wt = w + (size_t)t * kLbH * CIN + qk;
-#pragma unroll 4
-    for (int cb = 0; cb < CIN; cb += 16) {
-      const lb_bf16x4 a = *(const lb_bf16x4 *)(arow + cb);
+    const unsigned short *wrow = w + ((size_t)t * kLbH + 16 * v + qr) * CIN + qk;
+    const unsigned short *arow = in + (size_t)(qr + t) * is + qk;
+    // every weight fragment of the tap in flight at once (one memory
+    // latency per tap instead of one per few fragments)
+    lb_bf16x4 bw[CIN / 16];
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        const lb_bf16x4 b = *(const lb_bf16x4 *)(wt + (size_t)(16 * n + qr) * CIN + cb);
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+    for (int k = 0; k < CIN / 16; ++k) bw[k] = *(const lb_bf16x4 *)(wrow + 16 * k);
+#pragma unroll
+    for (int cb = 0; cb < CIN; cb += 16) {
+      const lb_bf16x4 b = bw[cb / 16];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const lb_bf16x4 a = *(const lb_bf16x4 *)(arow + (size_t)16 * m * is + cb);
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[m], 0, 0, 0);
       }
     }
   }
 }
+
+// Row and output channel of element j of acc[m] in lane `lane` of wave v.
+__device__ __forceinline__ int lb_row(int m, int lane, int j) { return 16 * m + 4 * (lane >> 4) + j; }
+__device__ __forceinline__ int lb_col(int v, int lane) { return 16 * v + (lane & 15); }
 
 // Hidden layer result -> LDS (bf16), lrelu, zero outside the side's
 // positions (row r is at position base + r).
 __device__ __forceinline__ void lb_store_hidden(unsigned short *h, const lb_f32x4 acc[4], int base,
                                                 int L) {
   const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-  for (int n = 0; n < 4; ++n)
+  const int co = lb_col(v, lane);
+  for (int m = 0; m < 4; ++m)
     for (int j = 0; j < 4; ++j) {
-      const int r = 16 * v + 4 * (lane >> 4) + j, co = 16 * n + (lane & 15);
+      const int r = lb_row(m, lane, j);
       const int p = base + r;
-      float y = acc[n][j];
+      float y = acc[m][j];
       y = y > 0.f ? y : 0.1f * y;
       h[r * kLbHS + co] = (p >= 0 && p < L) ? lb_bf16(y) : (unsigned short)0;
     }
@@ -181,14 +200,32 @@ template <class Put>
 __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_t soff, int p0,
                                         const LbLevels lev, LbLds &S, Put put) {
   const float *inv = net.inv[side];
-  // input rows r at positions p0 - 6 + r
-  for (int e = threadIdx.x; e < kLbRows * kLbCin; e += kLbBlock) {
-    const int r = e / kLbCin, ci = e % kLbCin;
+  // input rows r at positions p0 - 6 + r, as float4 runs of 4 cells of one
+  // field (12 cells per field): 24 per row and level
+  constexpr int kQ = kLbBand / 4, kN = kLbRows * kLbLev * kQ;
+  constexpr int kIt = (kN + kLbBlock - 1) / kLbBlock;
+  float4 xs[kIt];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {   // all loads first
+    const int q = threadIdx.x + k * kLbBlock;
+    const int r = q / (kLbLev * kQ), rem = q % (kLbLev * kQ);
+    const int lv = rem / kQ, c4 = rem % kQ;
     const int p = p0 - 6 + r;
-    const int lv = ci / kLbBand, fc = ci % kLbBand;
-    float x = 0.f;
-    if (p >= 0 && p < L) x = lb_level(lev, lv)[soff + (size_t)p * kLbBand + fc] * inv[fc / kLbCells];
-    S.x[r * kLbXS + ci] = lb_bf16(x);
+    xs[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < kN && p >= 0 && p < L)
+      xs[k] = *(const float4 *)(lb_level(lev, lv) + soff + (size_t)p * kLbBand + 4 * c4);
+  }
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const int q = threadIdx.x + k * kLbBlock;
+    if (q >= kN) break;
+    const int r = q / (kLbLev * kQ), rem = q % (kLbLev * kQ);
+    const int lv = rem / kQ, c4 = rem % kQ;
+    const float4 x = xs[k];
+    const float sc = inv[(4 * c4) / kLbCells];
+    const unsigned int lo = lb_bf16(x.x * sc) | ((unsigned int)lb_bf16(x.y * sc) << 16);
+    const unsigned int hi = lb_bf16(x.z * sc) | ((unsigned int)lb_bf16(x.w * sc) << 16);
+    *(uint2 *)(S.x + r * kLbXS + lv * kLbBand + 4 * c4) = make_uint2(lo, hi);
   }
   __syncthreads();
   lb_f32x4 acc[4];
@@ -200,19 +237,30 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
   __syncthreads();
   lb_layer<kLbH>(S.h2, kLbHS, net.w3[side], acc);
   const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-  for (int n = 0; n < 4; ++n)
+  const int co = lb_col(v, lane), f = co >> 3, c = co & 7;
+  if (c < kLbS - net.inner) return;
+  const float g = net.gain * net.scale[side][f];
+  for (int m = 0; m < 4; ++m)
     for (int j = 0; j < 4; ++j) {
-      const int r = 16 * v + 4 * (lane >> 4) + j, co = 16 * n + (lane & 15);
+      const int r = lb_row(m, lane, j);
       const int p = p0 + r;
-      if (r < kLbP && p < L && (co & 7) >= kLbS - net.inner) {
-        const int f = co >> 3, c = co & 7;
+      if (r < kLbP && p < L) {
         const float prev = lev.p3[soff + (size_t)p * kLbBand + f * kLbCells + c];
-        put(p, f, c, prev, net.gain * net.scale[side][f] * acc[n][j]);
+        put(p, f, c, prev, g * acc[m][j]);
       }
     }
 }
 
 __host__ __device__ inline int lb_tiles(int L) { return (L + kLbP - 1) / kLbP; }
+
+// New value of a strip cell: cpml is what the 8-cell CPML has just computed
+// there, prev its value before the step, d the network's gain * scale * output.
+__device__ __forceinline__ float lb_update(int form, float cpml, float prev, float d) {
+  if (form == kLbDirect) return prev + d;
+  const float y = cpml + d;
+  if (form == kLbResidual) return y;
+  return cpml >= 0.f ? fminf(fmaxf(y, 0.f), cpml) : fmaxf(fminf(y, 0.f), cpml);
+}
 
 }  // namespace bqk
 

@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kLbBlock) void k_lb_apply(LbNet net, KParams k, LbL
             float *F = f == 0 ? k.U1 : f == 1 ? k.W1 : f == 2 ? k.QX1 : f == 3 ? k.QZ1
                      : f == 4 ? k.TXX1 : f == 5 ? k.TZZ1 : f == 6 ? k.TXZ1 : k.PR1;
             float &x = F[(size_t)l * k.P + (size_t)(i + k.OFF)];
-            // residual: the CPML's update of this step is in x
-            x = (net.residual ? x : prev) + d;
+            // the CPML's update of this step is in x
+            x = lb_update(net.form, x, prev, d);
           });
 }
 
@@ -380,9 +380,10 @@ int grid_of(size_t n) { return (int)std::min<size_t>((n + kTB - 1) / kTB, 1 << 2
 
 extern "C" {
 
-int bq_lb_create(const float *weights, const float *scales, int residual, float gain, int inner,
+int bq_lb_create(const float *weights, const float *scales, int form, float gain, int inner,
                  bq_lb **out) {
-  if (!out || !weights || !scales || inner < 1 || inner > kLbS) return -1;
+  if (!out || !weights || !scales || inner < 1 || inner > kLbS || form < kLbDirect || form > kLbDamping)
+    return -1;
   if (!(gain >= 0.f && gain <= 16.f)) {
     g_lb_err = "bq_lb_create: gain outside [0, 16]";
     return -2;
@@ -412,7 +413,7 @@ int bq_lb_create(const float *weights, const float *scales, int residual, float 
     bq_lb_destroy(lb);
     return -101;
   }
-  lb->net.residual = residual != 0;
+  lb->net.form = form;
   lb->net.gain = gain;
   lb->net.inner = inner;
   for (int s = 0; s < 3; ++s) {

@@ -8,13 +8,16 @@
 2. Scales per side and field (RMS), then Adam on each side's network
    (--iters minibatches of 8 groups, cosine-decayed learning rate --lr),
    with a tenth of the groups held out for the validation loss.
-3. Runs --check held-out models (seeds from 10000) three ways - 40-cell PML
-   (the judge), 8-cell strip with the network, 8-cell plain PML - and prints
-   the reflected energy of the last two against the first at a few levels.
+3. Runs --check held-out models (seeds from 10000) several ways - 40-cell
+   PML (the judge), 8-cell strip with the network at each of --gains (in the
+   residual and, with --damping, the damping form), 8-cell plain PML - and
+   prints the reflected energy of the others against the judge.
 
 Writes the networks to --out (npz: weights, scales, loss history, check).
+With --load, steps 1 and 2 are skipped and a saved network is checked.
 
     python 2d-vqsp-fd-sim_amd/train_boundary.py --out weights.npz
+    python 2d-vqsp-fd-sim_amd/train_boundary.py --load weights.npz --damping --gains 1,0.5
 """
 from __future__ import annotations
 
@@ -36,20 +39,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def check(net, gains, n, steps, every, size=1000):
-    """Reflected energy of the learned boundary at each gain and of the plain
-    8-cell boundary against the 40-cell PML run on n held-out models."""
+def check(net, gains, n, steps, every, size=1000, forms=(False,)):
+    """Reflected energy of the learned boundary at each gain (and form: damping
+    False/True) and of the plain 8-cell boundary against the 40-cell PML run
+    on n held-out models."""
     out = []
     for k in range(n):
         seed = 10_000 + k
         wide = lib.Simulation(learned.training_job(seed, steps, 40, size, size))
         runs = {"plain8": lib.Simulation(learned.training_job(seed, steps, 8, size, size))}
         nets = []
-        for g in gains:
-            s = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
-            nets.append(net.with_gain(g))
-            nets[-1].attach(s)
-            runs[f"learned@{g:g}"] = s
+        for damp in forms:
+            for g in gains:
+                s = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
+                nets.append(net.with_gain(g, damping=damp))
+                nets[-1].attach(s)
+                runs[f"{'damping' if damp else 'learned'}@{g:g}"] = s
         acc = {m: np.zeros(2) for m in runs}
         done = 0
         for t in range(every, steps + 1, every):
@@ -87,7 +92,17 @@ def main():
                     help="strip cells nearest the interior that the network corrects (1..8)")
     ap.add_argument("--gains", default="1,0.5,0.25",
                     help="output gains to check on the held-out models")
+    ap.add_argument("--damping", action="store_true",
+                    help="also check the damping form of the residual network")
+    ap.add_argument("--load", default=None, help="check this saved network only")
     args = ap.parse_args()
+    gains = [float(g) for g in args.gains.split(",")]
+    forms = (False, True) if args.damping else (False,)
+    if args.load:
+        net = learned.Network.load(args.load)
+        res = check(net, gains, args.check, args.steps, 500, forms=forms)
+        print(json.dumps({"check": res}), flush=True)
+        return
     t0 = time.perf_counter()
     per_model = len(range(args.first, args.steps - 1, args.every))
     tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch,
@@ -100,8 +115,7 @@ def main():
     hist = learned.train(tr, learned.init_weights(args.seed), args.iters, args.lr, seed=args.seed, log=log)
     t_train = time.perf_counter() - t1
     net = learned.Network(tr.get_weights(), scales, residual=not args.direct, inner=args.inner)
-    gains = [float(g) for g in args.gains.split(",")]
-    res = check(net, gains, args.check, args.steps, 500) if args.check else []
+    res = check(net, gains, args.check, args.steps, 500, forms=forms) if args.check else []
     summary = {"groups": ng, "record_s": round(t_rec, 1), "train_s": round(t_train, 1),
                "iters": args.iters, "lr": args.lr, "residual": not args.direct,
                "inner": args.inner,

@@ -26,10 +26,12 @@
  * fastest; tap t multiplies the input at position p + t - 2. Input channel
  * level*96 + field*12 + cell (level 0 = oldest), output channel field*8 +
  * cell. Scales: [3][8], one per side and field; inputs are divided by them and
- * the outputs multiplied. Two forms: direct (the output times gain is the
- * strip's change over the step) and residual (the output times gain corrects
- * the value the 8-cell CPML has just computed for the strip cell; gain 0 is
- * the plain CPML).
+ * the outputs multiplied. Three forms: direct (0: the output times gain is
+ * the strip's change over the step), residual (1: the output times gain
+ * corrects the value the 8-cell CPML has just computed for the strip cell;
+ * gain 0 is the plain CPML) and damping (2: the residual correction limited
+ * to the interval between 0 and the CPML's value, so the network can only
+ * damp the strip, like a sponge whose per-cell factors it picks).
  */
 #ifndef BIOTFD_LEARNED_H
 #define BIOTFD_LEARNED_H
@@ -53,11 +55,11 @@ int bq_band_size(int mx, int mzp);
 int bq_band_extract(bq_handle *h, float *dst);
 
 /* Networks of the three sides from FP32 weights and scales (layout above),
- * converted to BF16 on the device that is current; residual 0/1 selects the
- * form, gain (0..16) scales the output, inner (1..8) is the number of strip
- * cells, counted from the interior, the output is applied to (the others
- * keep their update). */
-int bq_lb_create(const float *weights, const float *scales, int residual, float gain, int inner,
+ * converted to BF16 on the device that is current; form 0/1/2 (direct,
+ * residual, damping), gain (0..16) scales the output, inner (1..8) is the
+ * number of strip cells, counted from the interior, the output is applied to
+ * (the others keep their update). */
+int bq_lb_create(const float *weights, const float *scales, int form, float gain, int inner,
                  bq_lb **out);
 void bq_lb_destroy(bq_lb *lb);
 const char *bq_lb_last_error(void);

@@ -136,13 +136,14 @@ def test_batch_with_learned_boundary_matches_shots_run_alone():
     w = learned.init_weights(5)
     # small gains keep these untrained networks' runs finite over the steps
     nets = [learned.Network(w, np.full((3, 8), 0.8, np.float32), residual=False, gain=0.02),
-            learned.Network(w, np.full((3, 8), 1.3, np.float32), residual=True, gain=0.05, inner=3)]
+            learned.Network(w, np.full((3, 8), 1.3, np.float32), residual=True, gain=0.05, inner=3),
+            learned.Network(w, np.full((3, 8), 1.1, np.float32), residual=True, gain=1.0, damping=True)]
     js = [jobs.tiny_layered(steps=STEPS, tpml=8), jobs.medium_layered(steps=STEPS, tpml=8),
-          jobs.tiny_edges(steps=STEPS)]
+          jobs.tiny_layered(steps=STEPS, tpml=8), jobs.tiny_edges(steps=STEPS)]
 
     def make():
         sims = [lib.Simulation(j) for j in js]
-        for k, s in enumerate(sims[:2]):
+        for k, s in enumerate(sims[:3]):
             rng = np.random.default_rng(40 + k)
             for f in FIELDS:
                 s.set_field(f, (0.01 * rng.standard_normal((s.nz, s.nx))).astype(np.float32))

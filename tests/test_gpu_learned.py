@@ -130,12 +130,16 @@ def test_band_extract_matches_host_and_training_kernels_match_torch(residual, in
     tr.close()
 
 
-@pytest.mark.parametrize("residual,gain,inner", [(False, 1.0, 8), (True, 0.5, 8), (True, 2.0, 2)])
-def test_attached_network_replaces_only_the_strip(residual, gain, inner):
+@pytest.mark.parametrize("residual,gain,inner,damping",
+                         [(False, 1.0, 8, False), (True, 0.5, 8, False), (True, 2.0, 2, False),
+                          (True, 3.0, 8, True)])
+def test_attached_network_replaces_only_the_strip(residual, gain, inner, damping):
     """One step from a random state: every cell outside the 8-cell strip is
     bit-identical to the plain 8-cell PML run; the strip cells are the
     network's prediction from the band of the start level (direct form), or
-    the plain run's strip plus the network's correction (residual form)."""
+    the plain run's strip plus the network's correction (residual form), or
+    that sum limited to the interval between 0 and the plain run's value
+    (damping form)."""
     job = jobs.tiny_layered(steps=10, tpml=8)
     plain, run = lib.Simulation(job), lib.Simulation(job)
     _random_state(plain, 5)
@@ -143,7 +147,7 @@ def test_attached_network_replaces_only_the_strip(residual, gain, inner):
     band0 = learned.band_host(_host_fields(run), job.mx, job.mzp, job.tpml)
     w = _weights(3, last=10.0)
     scales = np.full((3, 8), 0.7, np.float32)
-    net = learned.Network(w, scales, residual=residual, gain=gain, inner=inner)
+    net = learned.Network(w, scales, residual=residual, gain=gain, inner=inner, damping=damping)
     net.attach(run)
     plain.run(1)
     run.run(1)
@@ -168,7 +172,13 @@ def test_attached_network_replaces_only_the_strip(residual, gain, inner):
             ii, ll = np.broadcast_to(ii, pred.shape), np.broadcast_to(ll, pred.shape)
             strip[ll, ii - pmll] = True
             got = a[ll, ii - pmll]
-            want = b[ll, ii - pmll] + (pred - prev) if residual else pred
+            cpml = b[ll, ii - pmll]
+            want = cpml + (pred - prev) if residual else pred
+            if damping:
+                want = np.where(cpml >= 0, np.clip(want, 0, np.maximum(cpml, 0)),
+                                np.clip(want, np.minimum(cpml, 0), 0))
+                # the limit binds on a good share of the cells at this gain
+                assert np.mean(np.abs(want - (cpml + pred - prev)) > 0) > 0.1
             outer = np.arange(8) < 8 - inner       # strip cells the network leaves alone
             want[:, outer] = b[ll, ii - pmll][:, outer]
             scale = np.abs(pred - prev).max() + 1e-30
