@@ -60,37 +60,62 @@ class Rock:
 
     def table_row(self) -> np.ndarray:
         """The 24 raw ``.MO`` entries of this rock (float64)."""
-        phi, rs, rf = self.phi, self.rho_s, self.rho_f
-        rho = (1.0 - phi) * rs + phi * rf
-        m = self.tort * rf / phi
-        d = rho * m - rf * rf
-        b = self.eta / self.perm
-        alpha = 1.0 - self.kdry / self.ks
-        M = 1.0 / (phi / self.kf + (alpha - phi) / self.ks)
-        lamc = self.kdry - 2.0 * self.mu / 3.0 + alpha * alpha * M
-        l2mc = lamc + 2.0 * self.mu
-        row = np.zeros(24)
-        v = {
-            "MU": self.mu,
-            "SIG_XX_1": l2mc, "SIG_XX_2": lamc, "SIG_XX_3": alpha * M,
-            "SIG_ZZ_1": lamc, "SIG_ZZ_2": l2mc, "SIG_ZZ_3": alpha * M,
-            "PRES_1": -alpha * M, "PRES_2": -alpha * M, "PRES_3": -M,
-            "REL1_U": m / d, "REL1_W": m / d,
-            "REL2_U": rf * b / d, "REL2_W": rf * b / d,
-            "REL3_U": rf / d, "REL3_W": rf / d,
-            "REL1_QX": rf / d, "REL1_QZ": rf / d,
-            "REL2_QX": rho / d, "REL2_QZ": rho / d,
-            "REL3_QX": rho * b / d, "REL3_QZ": rho * b / d,
-            "AUXIL_W": (lamc - alpha * alpha * M) / (l2mc - alpha * alpha * M),
-            "AUXIL_QZ": 2.0 * alpha * self.mu / (l2mc - alpha * alpha * M),
-        }
-        for k, val in v.items():
-            row[MO_INDEX[k]] = val
-        return row
+        return table_rows(np.array([self.phi]), np.array([self.kdry]), np.array([self.mu]),
+                          np.array([self.perm]), self.rho_s, self.rho_f, self.ks, self.kf,
+                          self.eta, self.tort)[0]
 
     def speeds(self):
         """(fast P, slow P, S) speeds in the no-friction limit, as calcul_velocity.F90."""
         return wave_speeds(self.table_row())
+
+
+def table_rows(phi, kdry, mu, perm, rho_s=RHO_S, rho_f=RHO_F, ks=K_S, kf=K_F, eta=ETA,
+               tort=TORT) -> np.ndarray:
+    """Raw ``.MO`` rows ``[n][24]`` (float64) of n rocks given as arrays of
+    porosity, dry bulk modulus, shear modulus and permeability."""
+    phi, kdry, mu, perm = (np.asarray(v, np.float64) for v in (phi, kdry, mu, perm))
+    rs, rf = rho_s, rho_f
+    rho = (1.0 - phi) * rs + phi * rf
+    m = tort * rf / phi
+    d = rho * m - rf * rf
+    b = eta / perm
+    alpha = 1.0 - kdry / ks
+    M = 1.0 / (phi / kf + (alpha - phi) / ks)
+    lamc = kdry - 2.0 * mu / 3.0 + alpha * alpha * M
+    l2mc = lamc + 2.0 * mu
+    rows = np.zeros((phi.size, 24))
+    v = {
+        "MU": mu,
+        "SIG_XX_1": l2mc, "SIG_XX_2": lamc, "SIG_XX_3": alpha * M,
+        "SIG_ZZ_1": lamc, "SIG_ZZ_2": l2mc, "SIG_ZZ_3": alpha * M,
+        "PRES_1": -alpha * M, "PRES_2": -alpha * M, "PRES_3": -M,
+        "REL1_U": m / d, "REL1_W": m / d,
+        "REL2_U": rf * b / d, "REL2_W": rf * b / d,
+        "REL3_U": rf / d, "REL3_W": rf / d,
+        "REL1_QX": rf / d, "REL1_QZ": rf / d,
+        "REL2_QX": rho / d, "REL2_QZ": rho / d,
+        "REL3_QX": rho * b / d, "REL3_QZ": rho * b / d,
+        "AUXIL_W": (lamc - alpha * alpha * M) / (l2mc - alpha * alpha * M),
+        "AUXIL_QZ": 2.0 * alpha * mu / (l2mc - alpha * alpha * M),
+    }
+    for k, val in v.items():
+        rows[:, MO_INDEX[k]] = val
+    return rows
+
+
+def max_fast_speed(rows: np.ndarray) -> float:
+    """Largest fast-P speed over table rows ``[n][24]`` (the expressions of
+    :func:`wave_speeds`, vectorized)."""
+    g = lambda k: rows[:, MO_INDEX[k]]
+    a11, a12, a22 = g("REL1_U"), -g("REL3_U"), g("REL2_QX")
+    b11, b12, b22 = g("SIG_XX_1"), -g("PRES_1"), -g("PRES_3")
+    t00 = a11 * b11 + a12 * b12
+    t01 = a11 * b12 + a12 * b22
+    t10 = a12 * b11 + a22 * b12
+    t11 = a12 * b12 + a22 * b22
+    tr = t00 + t11
+    delta = tr * tr + 4.0 * (t01 * t10 - t00 * t11)
+    return float(np.sqrt((tr + np.sqrt(delta)) / 2.0).max())
 
 
 def wave_speeds(row: np.ndarray):
@@ -166,13 +191,14 @@ def receivers_row(n: int, x0_cell: int, dx_cells: int, row: int):
     return [(x0_cell + k * dx_cells, row, x0_cell + k * dx_cells, row) for k in range(n)]
 
 
-def make_job(name: str, rocks: Sequence[Rock], jm: np.ndarray, h: float, steps: int,
+def make_job(name: str, rocks, jm: np.ndarray, h: float, steps: int,
              f0: float, x_source: float, z_source: float, receivers=(), dt: float | None = None,
              **kw) -> Job:
-    """Assemble a :class:`Job` whose run has exactly ``steps`` time steps."""
-    table = np.stack([r.table_row() for r in rocks])
+    """Assemble a :class:`Job` whose run has exactly ``steps`` time steps;
+    ``rocks`` is a sequence of :class:`Rock` or a raw table ``[JMNUM][24]``."""
+    table = rocks if isinstance(rocks, np.ndarray) else np.stack([r.table_row() for r in rocks])
     if dt is None:
-        vmax = max(r.speeds()[0] for r in rocks)
+        vmax = max_fast_speed(table)
         dt = stable_dt(h, vmax)
     dt = float(np.float32(dt))
     # MT is chosen so that INT(MT/DT)+1 == steps in default-real arithmetic.
@@ -213,11 +239,28 @@ def layered_job(name="LAYER", mx=4000, mz=2000, h=2.0, steps=5000, n_layers=10, 
     With ``field_std > 0`` porosity, Kdry and mu are multiplied by
     ``1 + field_std * g`` for a Gaussian-correlated ``g`` (correlation length
     ``corr_len`` metres). ``levels`` quantizes ``g`` to that many values so the
-    material table stays indexed (``levels**3 * n_layers`` entries at most).
+    material table stays indexed (``levels**3 * n_layers`` entries at most);
+    ``levels=None`` gives every cell its own material (JMNUM = cells).
     """
     rng = np.random.default_rng(seed)
     rocks = layered_rocks(n_layers, rng)
     lay = layer_index(mz, n_layers, rng)
+    if field_std > 0.0 and levels is None:
+        g = [gaussian_field((mz + 1, mx), corr_len / h, rng) for _ in range(3)]
+        base = [np.array([getattr(r, k) for r in rocks])[lay][:, None] for k in ("phi", "kdry", "mu", "perm")]
+        f = [1.0 + field_std * x for x in g]
+        table = table_rows((base[0] * f[0]).ravel(), (base[1] * f[1]).ravel(),
+                           (base[2] * f[2]).ravel(), np.broadcast_to(base[3], (mz + 1, mx)).ravel())
+        del g, f
+        jm = np.arange(1, (mz + 1) * mx + 1, dtype=np.int32).reshape(mz + 1, mx)
+        vmax = max_fast_speed(table)
+        dt = stable_dt(h, vmax)
+        xs = float(rng.uniform(0.25, 0.75) * mx * h)
+        n_rec = min(n_rec, mx)
+        step = max(1, mx // n_rec)
+        recs = receivers_row(n_rec, 1, step, 2)
+        return make_job(name, table, jm, h, steps, f0, xs, 0.0, recs, dt=dt, tpml=tpml,
+                        key_tlp=True, **kw)
     if field_std <= 0.0:
         jm = np.repeat((lay + 1)[:, None], mx, axis=1).astype(np.int32)
         table_rocks = rocks

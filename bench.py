@@ -40,13 +40,22 @@ B_HALF = 52
 # read once, 8 written once, 1 index (the other group's fields are consumed in
 # registers and LDS within the same sweep).
 B_FUSED_MIN = 68
+# Per-cell materials (every cell its own material): the index is replaced by
+# the 22 precombined coefficients a cell's update needs, 10 for the stress
+# half-step and 12 for the velocity half-step: 88 + 96 = 184 B per cell-step.
+B_STEP_CELL = 184
+B_HALF_CELL = 88
+B_FUSED_MIN_CELL = 64 + 88
 TIMING_STRIDE = 8     # kernel durations from HIP events around every 8th step
 
 
-def make_workload(steps: int, mx=8000, mz=4000):
+def make_workload(steps: int, mx=8000, mz=4000, per_cell=False):
+    """The roofline model: 10 seeded layers times a Gaussian-correlated 5 %
+    field on porosity, Kdry and mu, quantized to 16 levels per parameter
+    (indexed materials) or, with per_cell, one material per cell."""
     job = models.layered_job(name="ROOF", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=10, seed=2,
                              tpml=20, f0=20.0, n_rec=1000, field_std=0.05, corr_len=50.0,
-                             levels=16)
+                             levels=None if per_cell else 16)
     return job
 
 
@@ -178,7 +187,7 @@ def run_batched(args):
                      "frac": round(gcell * B_STEP / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not per_cell:
         # the reference, serial, on the median-sized shot: per-step time from a
         # 12- and a 2-step run, times the steps of a shot
         med = jobs[int(np.argsort(cells)[len(cells) // 2])]
@@ -361,6 +370,9 @@ def main():
     ap.add_argument("--mx", type=int, default=8000)
     ap.add_argument("--mz", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--materials", choices=("indexed", "cell"), default="indexed",
+                    help="roofline workload: quantized material table (default) or one "
+                         "material per cell (184 B per cell-step basis; no CPU baseline)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 1 GiB device copy (k_copy_probe) and the fused "
                          "tiles' access pattern without arithmetic (k_march_probe)")
@@ -380,7 +392,10 @@ def main():
 
     total = args.warmup + args.steps
     t0 = time.perf_counter()
-    job = make_workload(total, args.mx, args.mz)
+    per_cell = args.materials == "cell"
+    b_step, b_half, b_min = ((B_STEP_CELL, B_HALF_CELL, B_FUSED_MIN_CELL) if per_cell
+                             else (B_STEP, B_HALF, B_FUSED_MIN))
+    job = make_workload(total, args.mx, args.mz, per_cell)
     nz, nx = hostsetup.grid_shape(job)
     print(f"[bench] model ready: {nx}x{nz} cells, JMNUM={job.table.shape[0]} "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
@@ -404,16 +419,16 @@ def main():
     u = sim.field("U")
     if not np.isfinite(u).all():
         raise SystemExit("non-finite field after run")
-    # k_step advances the fused cells by a whole step (104 B each) and the
-    # other cells by the stress half-step (52 B each)
+    # k_step advances the fused cells by a whole step (104 B each, 184 with
+    # per-cell materials) and the other cells by the stress half-step (52 B, 88)
     dom_name = "k_step"
-    dom_bytes = B_STEP * kcells["k_fused"] + B_HALF * kcells["k_stress"]
+    dom_bytes = b_step * kcells["k_fused"] + b_half * kcells["k_stress"]
     dom_s = ktimes[dom_name] * 1e-3
     achieved = dom_bytes / dom_s / 1e9
-    min_bytes = B_FUSED_MIN * kcells["k_fused"] + B_HALF * kcells["k_stress"]
+    min_bytes = b_min * kcells["k_fused"] + b_half * kcells["k_stress"]
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tr_path):
+    if os.path.exists(tr_path) and not per_cell:
         with open(tr_path) as f:
             traffic = json.load(f).get(dom_name)
 
@@ -424,7 +439,7 @@ def main():
         march_gbs, _ = probes.probe_march(nx, nz, 20)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not per_cell:
         per_step = cpu_reference_baseline(job)
         if per_step and per_step > 0:
             cpu = {"value": round(cells / per_step / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1,
@@ -441,21 +456,23 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"layered+correlated Biot half-space {args.mx}x{args.mz}, dx=2 m, "
-                               f"TPML=20, free surface (stress imaging)",
+                               f"TPML=20, free surface (stress imaging), "
+                               f"{'one material per cell' if per_cell else 'materials quantized to 16 levels'}",
                    "grid_cells": cells, "nxc": nx, "nzc": nz, "jmnum": int(job.table.shape[0]),
-                   "b_alg_bytes_per_cell_step": B_STEP, "receivers": len(job.receivers)},
-        # whole step, all launches: 104 B x NXc x NZc per step over ms_per_step
-        "hbm_roofline_frac_step": round(gcell * 1e9 * B_STEP / (HBM_PEAK_GBS * 1e9), 4),
+                   "b_alg_bytes_per_cell_step": b_step, "receivers": len(job.receivers)},
+        # whole step, all launches: B_alg x NXc x NZc per step over ms_per_step
+        "hbm_roofline_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "kernel_ms": {n: round(t, 5) for n, t in ktimes.items()},
         "kernel_cells": kcells,
         "roofline": {"bound": "hbm", "kernel": dom_name,
-                     "basis": "104 B per fused cell-step + 52 B per other stress cell, "
+                     "basis": f"{b_step} B per fused cell-step + {b_half} B per other stress cell, "
                               "per k_step launch",
                      "algorithmic_bytes": int(dom_bytes),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
-        # the least k_step can move (68 B per fused cell-step): its rate
+        # the least k_step can move (68 B per fused cell-step, 152 with
+        # per-cell materials): its rate
         "k_step_min_bytes": {"bytes": int(min_bytes), "gbs": round(min_bytes / dom_s / 1e9, 1),
                              "frac": round(min_bytes / dom_s / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,

@@ -7,7 +7,10 @@ over all steps) on long runs, printing one JSON line per case:
   quantized materials) on its full grid for --steps-full steps;
 * the same model statistics on a 1000 x 500 grid for its full 8000 steps
   (--steps-long), so that the error growth over the workload's whole record
-  is stated at a size the binary64 oracle finishes in minutes.
+  is stated at a size the binary64 oracle finishes in minutes;
+* the learned-boundary workload's judge (40-cell PML, binary32 on the GPU)
+  and the runs it judges, against the binary64 oracle's 40-cell PML run, on
+  the smallest held-out shots (--judge-shots).
 
 Needs a GPU; the oracle runs on the host cores (OMP_NUM_THREADS).
 
@@ -29,7 +32,7 @@ sys.path[:0] = [os.path.join(ROOT, "2d-vqsp-fd-sim_amd"), os.path.join(ROOT, "or
 
 import bench  # noqa: E402
 import oracle  # noqa: E402
-from biotfd import hostsetup, lib  # noqa: E402
+from biotfd import hostsetup, lib, models  # noqa: E402
 
 FIELDS = ("U", "W", "QX", "QZ", "TXX", "TZZ", "TXZ", "P")
 COLS = ("U", "-W", "QX", "-QZ", "PRES", "-TXX")
@@ -69,11 +72,48 @@ def state(name, job):
     ref.close()
 
 
+def judge(n_shots, steps):
+    """The learned-boundary workload's judge against binary64: on the n
+    smallest of its 64 held-out shots (seeds from 20000), the receiver traces
+    (all six columns pooled, relative L2) of the 40-cell PML run on the GPU
+    (the judge the workload uses), of the 8-cell strip with the trained
+    network, of the plain 8-cell and of the plain 20-cell PML, each against
+    the binary64 oracle's 40-cell PML run."""
+    from biotfd import learned
+    wide = models.shot_set(64, steps, seed=20_000, tpml=40)
+    order = np.argsort([np.prod(hostsetup.grid_shape(j)) for j in wide])[:n_shots]
+    net = learned.Network.load()
+    for i in order:
+        i = int(i)
+        jobs = {t: models.shot_set(64, steps, seed=20_000, tpml=t)[i] for t in (40, 8, 20)}
+        ref = oracle.OracleSim(jobs[40], "f64")
+        t0 = time.perf_counter()
+        ref.run(steps)
+        tref = ref.traces()
+        ref.close()
+        runs = {"gpu_fp32_pml40": lib.Simulation(jobs[40]), "learned_strip8": lib.Simulation(jobs[8]),
+                "plain_pml8": lib.Simulation(jobs[8]), "plain_pml20": lib.Simulation(jobs[20])}
+        net.attach(runs["learned_strip8"])
+        out = {"case": f"learned-boundary judge, held-out shot {i}", "grid_pml40": list(hostsetup.grid_shape(jobs[40]))[::-1],
+               "steps": steps, "network": {"damping": net.damping, "gain": net.gain}, "oracle_s": 0.0,
+               "traces_rel_l2_vs_fp64_pml40": {}}
+        for k, sim in runs.items():
+            sim.run(steps)
+            out["traces_rel_l2_vs_fp64_pml40"][k] = rel_l2(sim.traces(), tref)
+            sim.close()
+        out["oracle_s"] = round(time.perf_counter() - t0, 1)
+        print(json.dumps(out), flush=True)
+    net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps-full", type=int, default=600)
     ap.add_argument("--steps-long", type=int, default=8000)
+    ap.add_argument("--judge-shots", type=int, default=2)
     args = ap.parse_args()
+    if args.judge_shots:
+        judge(args.judge_shots, 3000)
     if args.steps_long:
         state("roofline model statistics, 1000x500 grid", bench.make_workload(args.steps_long, 1000, 500))
     if args.steps_full:

@@ -4,7 +4,8 @@
   1500 steps, 100 receivers) in full: every snapshot and receiver file the
   reference writes, by SHA-256 digest (tests/golden/cfg0_digests.npz, written
   by the reference itself).
-* The layered half-space (4000 x 2000) and the roofline model (8000 x 4000)
+* The layered half-space (4000 x 2000) and the roofline model (8000 x 4000,
+  with quantized materials and with one material per cell, JMNUM = 32.2 M)
   on their full grids for a bounded number of steps, bit for bit against the
   binary32 oracle on the host: from random initial fields, so that every cell
   of every row class - side and bottom PML strips, their memories, the
@@ -94,18 +95,20 @@ def _cfg1_job(steps):
                               seed=1, tpml=20, f0=25.0, n_rec=1000)
 
 
-def _cfg2_job(steps):
+def _cfg2_job(steps, per_cell=False):
     import bench
-    return bench.make_workload(steps)
+    return bench.make_workload(steps, per_cell=per_cell)
 
 
 @pytest.mark.parametrize("which,steps,start", [
     ("layered 4000x2000", 24, "random"), ("layered 4000x2000", 300, "rest"),
     ("roofline 8000x4000", 12, "random"), ("roofline 8000x4000", 120, "rest"),
+    ("roofline 8000x4000 one material per cell", 6, "random"),
 ])
 def test_full_grid_bitexact_vs_oracle(which, steps, start):
     t0 = time.perf_counter()
-    job = _cfg1_job(steps) if which.startswith("layered") else _cfg2_job(steps)
+    job = (_cfg1_job(steps) if which.startswith("layered")
+           else _cfg2_job(steps, per_cell="per cell" in which))
     nz, nx = hostsetup.grid_shape(job)
     print(f"{which} ({start}): {nx}x{nz} cells, JMNUM={job.table.shape[0]}, {steps} steps, "
           f"{len(job.receivers)} receivers", flush=True)
