@@ -96,6 +96,10 @@ void build_pml(const GridDesc &g, const int32_t *jm, const float *tab, float r, 
 }
 
 // --- OPEN_SR_FILE: clamped cubic spline (open_sr_file.F90:121-226) ----------
+// The textbook interpolating cubic spline with prescribed end slopes: the
+// second derivatives from the tridiagonal system (forward elimination, back
+// substitution), evaluation after a bisection search for the bracketing
+// interval, as in Press et al., Numerical Recipes, sections 3.3 and 3.4.
 
 static int spl_locate(const std::vector<float> &xx, float x) {
   const int n = (int)xx.size();
@@ -201,7 +205,8 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
 // reference. The nine results a00 .. a22 are returned in a[0..8].
 
 // HIGDON_FD, higdon_fd.F90:1-46: product of two first-order Higdon operators
-// with speeds ALFA and BETA.
+// with speeds ALFA and BETA (Higdon's absorbing boundary conditions,
+// Math. Comp. 47, 1986), discretised with weight WB between time levels.
 static void higdon(float WB, float ALFA, float BETA, float DT, float H, double a[9]) {
   const double be1 = 1.0;
   const double be2 = (double)ALFA / (double)BETA;
@@ -223,7 +228,8 @@ static void higdon(float WB, float ALFA, float BETA, float DT, float H, double a
 }
 
 // HIGCE_FD, higdce_fd.F90:1-42: Higdon operator with speed BETA times a
-// first-order Clayton-Engquist one with speed ALFA.
+// first-order Clayton-Engquist one with speed ALFA (the paraxial absorbing
+// condition of Clayton and Engquist, BSSA 67, 1977).
 static void higdon_ce(float WB, float ALFA, float BETA, float DT, float H, double a[9]) {
   const double gp = (float)(ALFA * DT / H);
   const double gs = (float)(BETA * DT / H);
@@ -265,7 +271,8 @@ static void higdon_ce2(float WB, float VELH, float VELCE, float DT, float H, dou
   a[8] = -(qxt * rxt);
 }
 
-// OptAbsBc, OptAbsBc.F90:1-106: the 3x3 boundary stencil that minimises the
+// OptAbsBc, OptAbsBc.F90:1-106: the optimally absorbing boundary of Peng and
+// Toksoz (JASA 95, 1994): the 3x3 boundary stencil that minimises the
 // reflection of plane waves at the two incidence angles gp and gm, for the
 // normalised frequency wt = OMG*DT and wavenumber wx = OMG/VEL*H. r0 and r1
 // are the two stencils that each annihilate one angle; the result is
@@ -323,7 +330,8 @@ int nonref(int key, float H, float DT, float VEL, float VELS, float VELP, const 
     case 1:
       higdon(p.WB, VELP, VELS, DT, H, a);
       break;
-    case 2: {  // first-order one-way wave equation, upwind
+    case 2: {  // first-order one-way (transparent) wave equation, upwind: Reynolds'
+               // boundary (Geophysics 43, 1978) in its first-order form
       const float c = -(DT * VEL / H);
       ax[2] = 1.0f + c;
       ax[3] = 1.0f - c;
