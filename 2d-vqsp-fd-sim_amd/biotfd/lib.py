@@ -23,6 +23,7 @@ EXPORTS = (
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells",
+    "bq_set_sweep", "bq_get_sweep", "bq_kernel_stats",
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
     "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
     "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
@@ -80,6 +81,9 @@ def load(path: str = LIBPATH):
         "bq_enable_timing": ([p, C.c_int], C.c_int),
         "bq_kernel_times": ([p, p], C.c_int),
         "bq_kernel_cells": ([p, p], C.c_int),
+        "bq_set_sweep": ([p, C.c_int], C.c_int),
+        "bq_get_sweep": ([p], C.c_int),
+        "bq_kernel_stats": ([p, p, p, p], C.c_int),
         "bq_build_bottom_nr": ([p, p, p, p], C.c_int),
         "bq_nonref": ([C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, p, p],
                       C.c_int),
@@ -216,6 +220,30 @@ class Simulation:
         out = np.zeros(3, dtype=np.int64)
         self._chk(self.lib.bq_kernel_cells(self.h, _ptr(out)))
         return dict(zip(("k_fused", "k_stress", "k_velocity"), map(int, out)))
+
+    def set_sweep(self, steps: int):
+        """Steps per sweep bq_run may use: 2 (default, where the grid allows)
+        or 1."""
+        self._chk(self.lib.bq_set_sweep(self.h, int(steps)))
+
+    def sweep(self) -> int:
+        """Steps per sweep the next run uses (bq_get_sweep)."""
+        return int(self.lib.bq_get_sweep(self.h))
+
+    def kernel_stats(self) -> dict:
+        """Timing of the last timed run by launch kind (bq_kernel_stats)."""
+        ms = np.zeros(4, np.float64)
+        cells = np.zeros(4, np.int64)
+        sw = np.zeros(1, np.int32)
+        self._chk(self.lib.bq_kernel_stats(self.h, _ptr(ms), _ptr(cells), _ptr(sw)))
+        if sw[0] == 2:
+            names = ("k_sweep2", "k_rest_stress", "k_velocity", "k_edges")
+            cnames = ("region", "stress", "velocity")
+        else:
+            names = ("k_step", "k_velocity", "k_edges")
+            cnames = ("k_fused", "k_stress", "k_velocity")
+        return {"steps_per_sweep": int(sw[0]), "ms": dict(zip(names, map(float, ms))),
+                "cells": dict(zip(cnames, map(int, cells)))}
 
     def field(self, name: str) -> np.ndarray:
         out = np.zeros((self.nz, self.nx), dtype=np.float32)

@@ -85,6 +85,37 @@ __global__ __launch_bounds__(kStepBlock) void k_step(KParams k, Fused fu, Split 
   sample_receivers(k, b * kBlock + threadIdx.x, ln.sample_step, true);
 }
 
+// Two-step sweep of steps itile, itile+1 over the region: set A -> set C,
+// and the rim's level n+1 into the scratch set T.
+__global__ __launch_bounds__(kF2T, 2) void k_sweep2(KParams kac, Fused2 f2, F8 rim, int nregion,
+                                                    int itile) {
+  __shared__ Fused2Lds L;
+  fused2_block(kac, f2, rim, xcd_tile(blockIdx.x, nregion), itile, L);
+}
+
+// The stress half-step of the cells outside a two-step region (per-cell
+// blocks, then runs), then the receiver samples of step sample_step from the
+// current set; no LDS, so these blocks fit beside the two-step tiles.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_rest_stress(KParams k, Split sp, Launch1 ln, int itile) {
+  int b = blockIdx.x;
+  if (b < ln.nrest) {
+    int i, l;
+    if (rest_cell(sp, b * kBlock + threadIdx.x, &i, &l)) stress_cell(k, i, l, itile);
+    return;
+  }
+  b -= ln.nrest;
+  if (b < ln.nruns) {
+    int tile;
+    const int r = run_rect(sp, xcd_tile(b, ln.nruns), &tile);
+    stress_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+                   threadIdx.x & 63, itile);
+    return;
+  }
+  b -= ln.nruns;
+  sample_receivers(k, b * kBlock + threadIdx.x, ln.sample_step, true);
+}
+
 // ------------------------------------------------------------------- edges
 // Row 1, stress imaging (HPLANE_1_VELOCITY, hplane_1_velocity.F90:61-76):
 // INH_V_QX_INT_1_SI (inh_v_qx_int_1_si.F90:26-31) for all x first, then
@@ -285,6 +316,22 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   h->fused_cells = plan.fused_cells;
   h->stress_cells = plan.stress_cells;
   h->velocity_cells = plan.velocity_cells;
+  // two steps per sweep where the grid has a region for it: its partition and
+  // the scratch set (cleared, like the others)
+  const bq::Plan2 plan2 = bq::plan_cells2(g, k.OFF, kF2W, kF2R, kRun, z0, z0);
+  if (plan2.f2.nfx > 0) {
+    if (dalloc(h, &h->scratch_set, 8 * fsz) || hipStreamSynchronize(h->stream) != hipSuccess) {
+      create_fail(-101, h->err);
+      bq_destroy(h);
+      return -101;
+    }
+    h->f2 = plan2.f2;
+    h->ss2 = plan2.ss;
+    h->sv2 = plan2.sv;
+    h->region_cells = plan2.region_cells;
+    h->stress_cells2 = plan2.stress_cells;
+    h->velocity_cells2 = plan2.velocity_cells;
+  }
   *out = h;
   return 0;
 }
@@ -489,7 +536,31 @@ int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *
   h->kp.traces = tr;
   h->kp.MR = mr;
   h->mr = mr;
+  // two-step sweeps keep level n+1 only outside the region and in its rim:
+  // every sampled node must lie there
+  const bq::Fused2 &f = h->f2;
+  auto ok = [&](int x, int z) {
+    const bool in = f.nfx > 0 && x >= f.x0 && x < f.x1 && z >= f.z0 && z < f.z1;
+    const bool rim = x < f.x0 + 2 || x >= f.x1 - 2 || z < f.z0 + 2 || z >= f.z1 - 2;
+    return !in || rim;
+  };
+  h->rec_sweep_ok = true;
+  for (int j = 0; j < mr; ++j)
+    if (!ok(irecx[j], lrecx[j]) || !ok(irecz[j], lrecz[j]) || !ok(irecz[j], lrecx[j]))
+      h->rec_sweep_ok = false;
   return 0;
+}
+
+int bq_set_sweep(bq_handle *h, int steps) {
+  if (!h) return -1;
+  if (steps != 1 && steps != 2) return fail(h, -2, "bq_set_sweep: steps per sweep must be 1 or 2");
+  h->sweep = steps;
+  return 0;
+}
+
+int bq_get_sweep(bq_handle *h) {
+  if (!h) return -1;
+  return (h->sweep == 2 && h->f2.nfx > 0 && h->scratch_set && h->rec_sweep_ok && !h->lb) ? 2 : 1;
 }
 
 int bq_enable_timing(bq_handle *h, int every) {
@@ -516,10 +587,22 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   const int vgrid = vrest + h->sv.qcum[kMaxRuns];
   const int nsb = (h->mr + kBlock - 1) / kBlock;
   const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
+  // two steps per sweep (bq_get_sweep says when): the partition of the other cells
+  const bool two = bq_get_sweep(h) == 2;
+  Launch1 l2{};
+  l2.nrest = (h->ss2.cum[kMaxCells] + kBlock - 1) / kBlock;
+  l2.nruns = h->ss2.qcum[kMaxRuns];
+  const int vrest2 = (h->sv2.cum[kMaxCells] + kBlock - 1) / kBlock;
+  const int vgrid2 = vrest2 + h->sv2.qcum[kMaxRuns];
+  const int nregion = h->f2.nfx * h->f2.nfz;
+  F8 rim{};
+  for (int f = 0; f < 8; ++f)
+    rim.f[f] = h->scratch_set ? h->scratch_set + (size_t)f * h->P * g.NZ : nullptr;
   const int nsteps = itile_end - itile_begin + 1;
-  constexpr int kEv = 4;
+  // events of a timed sweep: before the first launch and after each of up to 7
+  constexpr int kEv = 8;
   const int every = h->timing;
-  const int ntimed = every ? (nsteps + every - 1) / every : 0;  // steps 0, every, 2*every, ...
+  const int ntimed = every ? (nsteps + every - 1) / every : 0;
   if (every) {
     const size_t need = (size_t)ntimed * kEv;
     while (h->ev.size() < need) {
@@ -528,43 +611,134 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       h->ev.push_back(e);
     }
   }
-  for (int it = itile_begin; it <= itile_end; ++it) {
-    const KParams k = step_params(h);
-    h->parity ^= 1;
-    // the samples of step it-1 ride along with the first launch of step it
-    ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
-    ln.sample_step = it - 1;
-    const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
+  std::vector<int> timed_kind;  // 1: single step, 2: two-step sweep, per timed slot
+  KParams last{};
+  int it = itile_begin;
+  while (it <= itile_end) {
     const int rel = it - itile_begin;
-    hipEvent_t *e = (every && rel % every == 0) ? &h->ev[(size_t)(rel / every) * kEv] : nullptr;
-    if (e) (void)hipEventRecord(e[0], h->stream);
-    if (g1)
-      hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu,
-                         h->ss, ln, it);
+    const bool sweep2 = two && it + 1 <= itile_end;
+    // timed: the sweeps that hold a step rel = 0, every, 2*every, ...
+    const bool timed = every && (rel % every == 0 || (sweep2 && (rel + 1) % every == 0)) &&
+                       (int)timed_kind.size() < ntimed;
+    hipEvent_t *e = timed ? &h->ev[timed_kind.size() * kEv] : nullptr;
+    if (e) {
+      timed_kind.push_back(sweep2 ? 2 : 1);
+      (void)hipEventRecord(e[0], h->stream);
+    }
+    if (!sweep2) {
+      const KParams k = step_params(h);
+      h->parity ^= 1;
+      // the samples of step it-1 ride along with the first launch of step it
+      ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
+      ln.sample_step = it - 1;
+      const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
+      if (g1)
+        hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), dim3(kStepBlock), 0, h->stream, k,
+                           h->fu, h->ss, ln, it);
+      if (e) (void)hipEventRecord(e[1], h->stream);
+      if (vgrid)
+        hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
+      if (e) (void)hipEventRecord(e[2], h->stream);
+      hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
+      if (e) (void)hipEventRecord(e[3], h->stream);
+      if (h->lb && bq::lb_after_step(h, k, it)) return fail(h, -100, "learned boundary launch failed");
+      last = k;
+      it += 1;
+      continue;
+    }
+    // steps it, it+1: A = current set, C = the other one, T = scratch set
+    const KParams kac = step_params(h);
+    float *A = (float *)kac.U0, *C = kac.U1, *T = h->scratch_set;
+    const KParams kat = params_sets(h, A, T), ktc = params_sets(h, T, C);
+    h->parity ^= 1;
+    hipLaunchKernelGGL(k_sweep2, dim3(nregion), dim3(kF2T), 0, h->stream, kac, h->f2, rim, nregion,
+                       it);
     if (e) (void)hipEventRecord(e[1], h->stream);
-    if (vgrid) hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
+    // the other cells, step it (A -> T), with the samples of step it-1 from A
+    l2.nsample = (it > itile_begin && kac.MR > 0) ? nsb : 0;
+    l2.sample_step = it - 1;
+    int gs = l2.nrest + l2.nruns + l2.nsample;
+    if (gs) hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, kat, h->ss2, l2, it);
     if (e) (void)hipEventRecord(e[2], h->stream);
-    hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
+    if (vgrid2)
+      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, kat, h->sv2, vrest2);
     if (e) (void)hipEventRecord(e[3], h->stream);
-    if (h->lb && bq::lb_after_step(h, k, it)) return fail(h, -100, "learned boundary launch failed");
-    if (it == itile_end && k.MR > 0)
-      hipLaunchKernelGGL(k_sample, dim3(nsb), dim3(kBlock), 0, h->stream, k, it);
+    hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, kat);
+    if (e) (void)hipEventRecord(e[4], h->stream);
+    // step it+1 (T -> C), with the samples of step it from T
+    l2.nsample = kac.MR > 0 ? nsb : 0;
+    l2.sample_step = it;
+    gs = l2.nrest + l2.nruns + l2.nsample;
+    if (gs)
+      hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, ktc, h->ss2, l2, it + 1);
+    if (e) (void)hipEventRecord(e[5], h->stream);
+    if (vgrid2)
+      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, ktc, h->sv2, vrest2);
+    if (e) (void)hipEventRecord(e[6], h->stream);
+    hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, ktc);
+    if (e) (void)hipEventRecord(e[7], h->stream);
+    last = ktc;
+    it += 2;
   }
+  if (last.MR > 0)
+    hipLaunchKernelGGL(k_sample, dim3(nsb), dim3(kBlock), 0, h->stream, last, itile_end);
   HIPCHK(h, hipGetLastError());
   h->next_step = itile_end + 1;
-  if (every) {
+  h->last_sweep = two ? 2 : 1;
+  if (every && !timed_kind.empty()) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    double acc[kEv - 1] = {0, 0, 0};
-    for (int s = 0; s < ntimed; ++s) {
-      for (int q = 0; q < kEv - 1; ++q) {
-        float ms = 0;
-        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[(size_t)s * kEv + q], h->ev[(size_t)s * kEv + q + 1]));
-        acc[q] += ms;
+    auto dt = [&](size_t slot, int a, int b) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, h->ev[slot * kEv + a], h->ev[slot * kEv + b]);
+      return (double)ms;
+    };
+    double s1[3] = {0, 0, 0}, s2[4] = {0, 0, 0, 0};
+    int n1 = 0, n2 = 0;
+    for (size_t q = 0; q < timed_kind.size(); ++q) {
+      if (timed_kind[q] == 1) {
+        for (int c = 0; c < 3; ++c) s1[c] += dt(q, c, c + 1);
+        ++n1;
+      } else {
+        s2[0] += dt(q, 0, 1);                          // the two-step sweep
+        s2[1] += dt(q, 1, 2) + dt(q, 4, 5);            // other cells, stress (two launches)
+        s2[2] += dt(q, 2, 3) + dt(q, 5, 6);            // velocity
+        s2[3] += dt(q, 3, 4) + dt(q, 6, 7);            // edges
+        ++n2;
       }
     }
-    for (int q = 0; q < kEv - 1; ++q) h->ktime[q] = acc[q] / ntimed;
-    h->timed_steps = ntimed;
+    if (n1)
+      for (int c = 0; c < 3; ++c) h->ktime[c] = s1[c] / n1;
+    if (n2) {
+      h->ktime2[0] = s2[0] / n2;
+      for (int c = 1; c < 4; ++c) h->ktime2[c] = s2[c] / (2.0 * n2);
+    }
+    h->timed_steps = n1 + n2;
+    h->last_sweep = n2 ? 2 : 1;
   }
+  return 0;
+}
+
+// Durations (ms per launch) and cells of the last timed run. steps_per_sweep
+// = 2: ms = {two-step sweep, stress of the other cells, velocity, edges},
+// cells = {region, other stress cells, other velocity cells, 0};
+// 1: ms = {k_step, k_velocity, k_edges, 0}, cells = {fused, stress, velocity, 0}.
+int bq_kernel_stats(bq_handle *h, double ms[4], long long cells[4], int *steps_per_sweep) {
+  if (!h || !ms || !cells || !steps_per_sweep) return -1;
+  if (!h->timed_steps) return fail(h, -6, "no timed run");
+  *steps_per_sweep = h->last_sweep;
+  if (h->last_sweep == 2) {
+    for (int c = 0; c < 4; ++c) ms[c] = h->ktime2[c];
+    cells[0] = h->region_cells;
+    cells[1] = h->stress_cells2;
+    cells[2] = h->velocity_cells2;
+  } else {
+    for (int c = 0; c < 3; ++c) ms[c] = h->ktime[c];
+    ms[3] = 0;
+    cells[0] = h->fused_cells;
+    cells[1] = h->stress_cells;
+    cells[2] = h->velocity_cells;
+  }
+  cells[3] = 0;
   return 0;
 }
 
@@ -579,7 +753,13 @@ int bq_kernel_cells(bq_handle *h, long long out[3]) {
 int bq_kernel_times(bq_handle *h, double out[3]) {
   if (!h || !out) return -1;
   if (!h->timed_steps) return fail(h, -6, "no timed run");
-  for (int q = 0; q < 3; ++q) out[q] = h->ktime[q];
+  if (h->last_sweep == 2) {
+    out[0] = h->ktime2[0];
+    out[1] = h->ktime2[2];
+    out[2] = h->ktime2[3];
+  } else {
+    for (int q = 0; q < 3; ++q) out[q] = h->ktime[q];
+  }
   return 0;
 }
 
@@ -617,9 +797,14 @@ int bq_get_field(bq_handle *h, int field_id, float *host) {
 
 int bq_set_field(bq_handle *h, int field_id, const float *host) {
   if (!h || !host) return fail(h, -1, "null argument");
-  // both sets: cells no step writes must hold the same value in each
-  for (int which = 0; which < 2; ++which) {
-    float *d = field_ptr(h, field_id, which);
+  // every set (the scratch set of two-step sweeps too): cells no step writes
+  // must hold the same value in each
+  for (int which = 0; which < 3; ++which) {
+    if (which == 2 && !h->scratch_set) break;
+    float *d = which < 2 ? field_ptr(h, field_id, which)
+                         : (field_id >= 0 && field_id < 8
+                                ? h->scratch_set + (size_t)field_id * h->P * h->g.NZ
+                                : nullptr);
     if (!d) return fail(h, -8, "bad field id");
     HIPCHK(h, hipMemcpy2DAsync(d, (size_t)h->P * 4, host, (size_t)h->g.NX * 4, (size_t)h->g.NX * 4,
                                h->g.NZ, hipMemcpyHostToDevice, h->stream));

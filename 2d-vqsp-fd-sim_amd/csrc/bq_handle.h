@@ -4,6 +4,7 @@
 #include <string>
 #include <vector>
 
+#include "bq_fused2.h"
 #include "bq_kernels.h"
 
 struct bq_handle {
@@ -14,7 +15,17 @@ struct bq_handle {
   KParams kp{};
   std::vector<void *> allocs;
   float *fields[2] = {nullptr, nullptr};  // two full field sets, current and next
-  int parity = 0;                         // steps done, mod 2
+  int parity = 0;                         // which of the two is current
+  // two steps per sweep: the region, the partition of the other cells, the
+  // scratch set holding level n+1 of those cells and of the region's rim
+  float *scratch_set = nullptr;
+  bq::Fused2 f2{};
+  Split ss2{}, sv2{};
+  long long region_cells = 0, stress_cells2 = 0, velocity_cells2 = 0;
+  int sweep = 1;            // steps per sweep allowed (bq_set_sweep)
+  bool rec_sweep_ok = true; // every receiver node outside the region or in its rim
+  int last_sweep = 1;       // steps per sweep of the last bq_run
+  double ktime2[4] = {0, 0, 0, 0};  // two-step sweep, other stress, velocity, edges (ms)
   std::vector<int32_t> jm_host;
   std::vector<float> tab_host;
   bool model_ok = false, pml_ok = false, src_ok = false;
@@ -62,3 +73,16 @@ inline KParams params_for(const bq_handle *h, int parity) {
 }
 
 inline KParams step_params(const bq_handle *h) { return params_for(h, h->parity); }
+
+// Kernel parameters reading set src and writing set dst (8 fields each).
+inline KParams params_sets(const bq_handle *h, const float *src, float *dst) {
+  KParams k = h->kp;
+  const size_t fsz = (size_t)h->P * h->g.NZ;
+  const float **f0[8] = {&k.U0, &k.W0, &k.QX0, &k.QZ0, &k.TXX0, &k.TZZ0, &k.TXZ0, &k.PR0};
+  float **f1[8] = {&k.U1, &k.W1, &k.QX1, &k.QZ1, &k.TXX1, &k.TZZ1, &k.TXZ1, &k.PR1};
+  for (int f = 0; f < 8; ++f) {
+    *f0[f] = src + f * fsz;
+    *f1[f] = dst + f * fsz;
+  }
+  return k;
+}

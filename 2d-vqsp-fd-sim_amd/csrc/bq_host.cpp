@@ -197,6 +197,68 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   return p;
 }
 
+// Two steps per sweep. The interior formulas hold for stresses at columns
+// [3, MX-2], rows [3, MZ'-2] and for velocities at columns [4, MX-2], rows
+// [2, MZ'-2]. A two-step tile forms level n+1 stresses 6 cells and
+// velocities 4 cells around its own cells, so the region is
+// [x0, x1) x [9, MZ'-7) with x0 >= 9 and x1 <= MX-7; it reads level-n
+// velocities 8 cells around, inside [1, MX] x [1, MZ']. Columns: x0 is the
+// first 128-byte aligned column >= max(9, fx), and x1 = x0 + nfx * fw2.
+// Outside the region:
+//   runs      [fx, xe) above the region (stress rows 5.., velocity rows 2..)
+//             and below it (to MZ'-1), and beside it on the 64-column grid
+//   per-cell  left strip [PMLL, fx), the surface rows, the bottom rows from
+//             MZ'-1, and the right strip from xe (from max(x1, xe) beside
+//             the region)
+Plan2 plan_cells2(const GridDesc &g, int off, int fw2, int fr2, int nr, int zs, int zv) {
+  Plan2 p{};
+  const int PX1 = g.PMLXH + 1, ZS1 = g.PMLZH + 1, ZV1 = g.PMLZH;
+  int fx = 5;
+  while ((fx + off) % 32 != 0) ++fx;
+  const int ntx = g.MX - 2 >= fx ? (g.MX - 2 - fx + 1) / 64 : 0;
+  const int xe = fx + 64 * ntx;
+  // the tiles' first own column on a 128-byte boundary, so that each
+  // wavefront's loads of a row are whole cache lines
+  int x0 = fx > 9 ? fx : 9;
+  while ((x0 + off) % 32 != 0) ++x0;
+  Fused2 &f = p.f2;
+  f.x0 = x0;
+  f.nfx = g.MX - 7 > x0 ? (g.MX - 7 - x0) / fw2 : 0;
+  f.x1 = x0 + fw2 * f.nfx;
+  f.z0 = 9;
+  f.z1 = g.MZ - 7;
+  f.fr = fr2;
+  f.nfz = (f.nfx > 0 && f.z1 > f.z0 && ntx > 0) ? (f.z1 - f.z0 + fr2 - 1) / fr2 : 0;
+  if (f.nfz == 0) {
+    f.nfx = 0;
+    return p;  // no region: the caller keeps single steps
+  }
+  p.region_cells = (long long)(f.x1 - f.x0) * (f.z1 - f.z0);
+  // beside the region: runs on the 64-column grid from fx ([fx, x0l) on the
+  // left, [x1r, xe) on the right), per-cell columns in the gaps [x0l, x0)
+  // and [x1, x1r)
+  const int x0l = fx + (x0 - fx) / 64 * 64;
+  int x1r = fx + (f.x1 - fx + 63) / 64 * 64;
+  if (x1r > xe) x1r = xe;
+  if (x1r < f.x1) x1r = f.x1;
+  const int xr = x1r > xe ? x1r : xe;
+  const Rect rs[4] = {{fx, xe, 5, f.z0}, {fx, xe, f.z1, g.MZ - 1}, {fx, x0l, f.z0, f.z1},
+                      {x1r, xe, f.z0, f.z1}};
+  const Rect cs[8] = {{g.PMLL, fx, zs, ZS1},    {fx, PX1, zs, 5},        {fx, PX1, g.MZ - 1, ZS1},
+                      {xe, PX1, 5, f.z0},        {xe, PX1, f.z1, g.MZ - 1}, {xr, PX1, f.z0, f.z1},
+                      {f.x1, x1r, f.z0, f.z1},   {x0l, x0, f.z0, f.z1}};
+  p.ss = make_split(rs, 4, cs, 8, nr);
+  const Rect rv[4] = {{fx, xe, 2, f.z0}, {fx, xe, f.z1, g.MZ - 1}, {fx, x0l, f.z0, f.z1},
+                      {x1r, xe, f.z0, f.z1}};
+  const Rect cv[8] = {{g.PMLL, fx, zv, ZV1},    {fx, PX1, zv, 2},        {fx, PX1, g.MZ - 1, ZV1},
+                      {xe, PX1, 2, f.z0},        {xe, PX1, f.z1, g.MZ - 1}, {xr, PX1, f.z0, f.z1},
+                      {f.x1, x1r, f.z0, f.z1},   {x0l, x0, f.z0, f.z1}};
+  p.sv = make_split(rv, 4, cv, 8, nr);
+  p.stress_cells = split_cells(p.ss);
+  p.velocity_cells = split_cells(p.sv);
+  return p;
+}
+
 // --------------------------------------------- bottom non-reflecting boundary
 //
 // The coefficient generators NONREF calls compute in binary64 from binary32

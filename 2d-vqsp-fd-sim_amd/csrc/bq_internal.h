@@ -38,8 +38,8 @@ struct Rect {
   int x0, x1, z0, z1;
 };
 
-constexpr int kMaxRuns = 3;   // run rectangles of one half-step
-constexpr int kMaxCells = 6;  // per-cell rectangles of one half-step
+constexpr int kMaxRuns = 4;   // run rectangles of one half-step
+constexpr int kMaxCells = 8;  // per-cell rectangles of one half-step
 
 // The cells of one half-step outside the fused tiles. Runs: up to kMaxRuns
 // rectangles of 64-column strips [qx0, qx0 + 64*qntx) x [qz0, qz1), in blocks
@@ -72,6 +72,25 @@ struct Plan {
 // (the rows outside these are the kernels' own edge rows). off = -PMLL; fw,
 // fr: fused tile width and height; nr: rows per wavefront run.
 Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv);
+
+// Two-step region: own cells [x0, x1) x [z0, z1) in nfx x nfz tiles of the
+// tile width and fr rows (the last row of tiles clipped); nfx = 0: none.
+struct Fused2 {
+  int x0, x1, z0, z1, nfx, nfz, fr;
+};
+
+struct Plan2 {
+  Fused2 f2;
+  Split ss, sv;  // stress and velocity half-steps of every cell outside the region
+  long long region_cells, stress_cells, velocity_cells;
+};
+
+// Partition for two steps per sweep: the region's cells are advanced twice by
+// the two-step tiles of width fw2 and height fr2; every other cell (the band
+// of interior cells around the region, the PML strips, the surface and bottom
+// rows) by the per-cell and run kernels, one step at a time. The region keeps
+// 6 cells of interior-formula cells on every side (its halos reach that far).
+Plan2 plan_cells2(const GridDesc &g, int off, int fw2, int fr2, int nr, int zs, int zv);
 
 // NONREF inputs from MODULE CONTROL_DATA: OMG as READ_OPEN leaves it (the
 // namelist value / 2 pi, read_open.F90:188), WB, THPPLE, THSSLE.

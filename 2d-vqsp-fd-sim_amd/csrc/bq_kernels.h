@@ -648,8 +648,102 @@ constexpr int kLW = kFW + 8;   // old-velocity LDS row: columns c-4 .. c+kFW+3
 constexpr int kSW = kFW + 4;   // new-stress LDS row: columns c-2 .. c+kFW+1
 constexpr int kStepBlock = kFW;  // threads of a k_step block
 
-// New TXX, TZZ, TXZ, PRES at LDS column q (global column i) of row s, from the
-// old-velocity rings (same expressions as stress_run).
+// New TXX, TZZ, TXZ, PRES at LDS column q (global column i) of row s, from
+// rings of old-velocity rows (row r of U in slot r % DU, of W in r % DW, of
+// QZ in r % DZ, of QX in r % DQ; LW floats per row) - the expressions of
+// stress_run. jm: the cell's material; itile: the step (source term).
+template <int DU, int DW, int DZ, int DQ, int LW>
+__device__ __forceinline__ void ring_stress_t(const KParams &k, const float (*sU)[LW],
+                                              const float (*sW)[LW], const float (*sZ)[LW],
+                                              const float (*sQ)[LW], int s, int q, int i,
+                                              float txx, float tzz, float txz, float pr,
+                                              const float4 s0, const float4 s1, const float4 s2,
+                                              int itile, float *o) {
+  const int u0s = s % DU, w0s = s % DW, z0s = s % DZ, qs = s % DQ;
+  const float um2 = sU[(s - 2) % DU][q], um1 = sU[(s - 1) % DU][q], u0 = sU[u0s][q];
+  const float up1 = sU[(s + 1) % DU][q];
+  const float uxm1 = sU[u0s][q - 1], uxp1 = sU[u0s][q + 1], uxp2 = sU[u0s][q + 2];
+  const float wm1 = sW[(s - 1) % DW][q], w0 = sW[w0s][q], wp1 = sW[(s + 1) % DW][q];
+  const float wp2 = sW[(s + 2) % DW][q];
+  const float wxm2 = sW[w0s][q - 2], wxm1 = sW[w0s][q - 1], wxp1 = sW[w0s][q + 1];
+  const float zm1 = sZ[(s - 1) % DZ][q], z0 = sZ[z0s][q], zp1 = sZ[(s + 1) % DZ][q];
+  const float zp2 = sZ[(s + 2) % DZ][q];
+  const float qxm1 = sQ[qs][q - 1], qx0 = sQ[qs][q], qxp1 = sQ[qs][q + 1], qxp2 = sQ[qs][q + 2];
+  const float AH = k.AH, BH = k.BH, BC = k.BC, C = k.CH, DT = k.DT;
+  const float exz = AH * (up1 - um2 + wxp1 - wxm2) + BH * (u0 - um1 + w0 - wxm1);
+  const float exx = AH * (uxp2 - uxm1) + BH * (uxp1 - u0);
+  const float qxx = AH * (qxp2 - qxm1) + BH * (qxp1 - qx0);
+  float ezz = (C * (wp1 - w0));
+  float qzz = (C * (zp1 - z0));
+  ezz = AH * (wp2 - wm1) + BC * ezz;
+  qzz = AH * (zp2 - zm1) + BC * qzz;
+  const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
+  const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
+  const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
+  o[2] = txz + DT * (s0.x * exz);
+  if (i == k.LIN_X && s == k.LIN_Z) {
+    const float2 ds = k.dsrc[itile - 1];
+    o[0] = txx + DT * (rxx - ds.x);
+    o[1] = tzz + DT * (rzz - ds.x);
+    o[3] = pr + DT * (rpp + ds.y);
+  } else {
+    o[0] = txx + DT * rxx;
+    o[1] = tzz + DT * rzz;
+    o[3] = pr + DT * rpp;
+  }
+}
+
+// ring_stress_t with the material's stress table entries read here.
+template <int DU, int DW, int DZ, int DQ, int LW>
+__device__ __forceinline__ void ring_stress(const KParams &k, const float (*sU)[LW],
+                                            const float (*sW)[LW], const float (*sZ)[LW],
+                                            const float (*sQ)[LW], int s, int q, int i, float txx,
+                                            float tzz, float txz, float pr, int jm, int itile,
+                                            float *o) {
+  const float4 s0 = k.stab[3 * jm], s1 = k.stab[3 * jm + 1], s2 = k.stab[3 * jm + 2];
+  ring_stress_t<DU, DW, DZ, DQ, LW>(k, sU, sW, sZ, sQ, s, q, i, txx, tzz, txz, pr, s0, s1, s2,
+                                    itile, o);
+}
+
+// New QX, U, QZ, W of one interior cell (INH_QX_INT, INH_V_UV_INT, INH_QZ_INT,
+// INH_V_W_INT: the expressions of velocity_run). X, XZ, PP: the new TXX, TXZ,
+// PRES of the cell's row at columns i-2 .. i+2 (element 2 is column i);
+// xz[0..3]: new TXZ of rows l-1 .. l+2, zz[0..3] and pp[0..3]: new TZZ and
+// PRES of rows l-2 .. l+1 at column i; q, u, qz, w: the old velocities.
+// o = {QX, U, QZ, W}.
+__device__ __forceinline__ void ring_velocity_t(const KParams &k, const float *X, const float *XZ,
+                                                const float *PP, float xz0, float xz1, float xz2,
+                                                float xz3, float zz0, float zz1, float zz2,
+                                                float zz3, float pp0, float pp1, float pp2,
+                                                float pp3, float q, float u, float qz, float w,
+                                                const float4 v0, const float4 v1, const float4 v2,
+                                                float *o) {
+  const float A = k.A, B = k.B;
+  const float xm2 = X[0], xm1 = X[1], x0 = X[2], xp1 = X[3];
+  const float xzm1 = XZ[1], xzc = XZ[2], xzp1 = XZ[3], xzp2 = XZ[4];
+  const float pm2 = PP[0], pm1 = PP[1], p0 = PP[2], pp1x = PP[3];
+  // TXZ(l-1..l+2) = xz0..xz3, TZZ and PRES (l-2..l+1) = zz0..zz3, pp0..pp3 at column i
+  const float sx = A * (xp1 - xm2 + xz3 - xz0) + B * (x0 - xm1 + xz2 - xz1);
+  const float px = A * (pp1x - pm2) + B * (p0 - pm1);
+  o[0] = v0.x * q - v0.y * sx - v0.z * px;
+  o[1] = u + v0.w * q + v1.x * sx + v1.y * px;
+  const float sz = A * (zz3 - zz0 + xzp2 - xzm1) + B * (zz2 - zz1 + xzp1 - xzc);
+  const float pz = A * (pp3 - pp0) + B * (pp2 - pp1);
+  o[2] = v1.z * qz - v1.w * sz - v2.x * pz;
+  o[3] = w + v2.y * qz + v2.z * sz + v2.w * pz;
+}
+
+// ring_velocity_t with the material's velocity table entries read here.
+__device__ __forceinline__ void ring_velocity(const KParams &k, const float *X, const float *XZ,
+                                              const float *PP, float xz0, float xz1, float xz2,
+                                              float xz3, float zz0, float zz1, float zz2, float zz3,
+                                              float pp0, float pp1, float pp2, float pp3, float q,
+                                              float u, float qz, float w, int jm, float *o) {
+  const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
+  ring_velocity_t(k, X, XZ, PP, xz0, xz1, xz2, xz3, zz0, zz1, zz2, zz3, pp0, pp1, pp2, pp3, q, u,
+                  qz, w, v0, v1, v2, o);
+}
+
 __device__ __forceinline__ void fused_stress(const KParams &k, const float (*sU)[kLW],
                                              const float (*sW)[kLW], const float (*sZ)[kLW],
                                              const float (*sQ)[kLW], int s, int q, int i,

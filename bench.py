@@ -370,6 +370,9 @@ def main():
     ap.add_argument("--mx", type=int, default=8000)
     ap.add_argument("--mz", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep", type=int, choices=(1, 2), default=1,
+                    help="roofline workload: steps per sweep (1, default: one step per launch "
+                         "sequence; 2: the interior two steps per pass, bq_set_sweep)")
     ap.add_argument("--materials", choices=("indexed", "cell"), default="indexed",
                     help="roofline workload: quantized material table (default) or one "
                          "material per cell (184 B per cell-step basis; no CPU baseline)")
@@ -400,7 +403,8 @@ def main():
     print(f"[bench] model ready: {nx}x{nz} cells, JMNUM={job.table.shape[0]} "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     sim = lib.Simulation(job)
-    print("[bench] uploaded", file=sys.stderr, flush=True)
+    sim.set_sweep(args.sweep)
+    print(f"[bench] uploaded; {sim.sweep()} step(s) per sweep", file=sys.stderr, flush=True)
     sim.run(args.warmup)
     print("[bench] warmup done", file=sys.stderr, flush=True)
     # HIP events around the launches of every 8th step of the timed region
@@ -410,8 +414,8 @@ def main():
     sim.run(args.steps, sync=False)
     sim.sync()
     t2 = time.perf_counter()
-    ktimes = sim.kernel_times()   # ms per launch of each kernel
-    kcells = sim.kernel_cells()   # cells per launch of the half-step kernels
+    stats = sim.kernel_stats()   # ms per launch and cells of each kind of launch
+    ktimes, kcells, sweep = stats["ms"], stats["cells"], stats["steps_per_sweep"]
     elapsed = t2 - t1
     ms_step = elapsed * 1e3 / args.steps
     cells = nx * nz
@@ -419,18 +423,31 @@ def main():
     u = sim.field("U")
     if not np.isfinite(u).all():
         raise SystemExit("non-finite field after run")
-    # k_step advances the fused cells by a whole step (104 B each, 184 with
-    # per-cell materials) and the other cells by the stress half-step (52 B, 88)
-    dom_name = "k_step"
-    dom_bytes = b_step * kcells["k_fused"] + b_half * kcells["k_stress"]
+    if sweep == 2:
+        # k_sweep2 advances the region's cells by two whole steps (2 x 104 B
+        # each, 184 with per-cell materials) in one pass; it moves at least
+        # 68 B (152) per region cell, once for both steps
+        dom_name = "k_sweep2"
+        dom_bytes = 2 * b_step * kcells["region"]
+        min_bytes = b_min * kcells["region"]
+        basis = (f"2 x {b_step} B per region cell (two steps per k_sweep2 launch); the other "
+                 f"cells' launches are timed separately")
+    else:
+        # k_step advances the fused cells by a whole step (104 B each, 184 with
+        # per-cell materials) and the other cells by the stress half-step (52 B, 88)
+        dom_name = "k_step"
+        dom_bytes = b_step * kcells["k_fused"] + b_half * kcells["k_stress"]
+        min_bytes = b_min * kcells["k_fused"] + b_half * kcells["k_stress"]
+        basis = f"{b_step} B per fused cell-step + {b_half} B per other stress cell, per k_step launch"
     dom_s = ktimes[dom_name] * 1e-3
     achieved = dom_bytes / dom_s / 1e9
-    min_bytes = b_min * kcells["k_fused"] + b_half * kcells["k_stress"]
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tr_path) and not per_cell:
         with open(tr_path) as f:
             traffic = json.load(f).get(dom_name)
+    if traffic is not None:
+        traffic = int(traffic)
 
     copy_gbs = march_gbs = None
     if args.copy_probe:
@@ -462,17 +479,17 @@ def main():
                    "b_alg_bytes_per_cell_step": b_step, "receivers": len(job.receivers)},
         # whole step, all launches: B_alg x NXc x NZc per step over ms_per_step
         "hbm_roofline_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
+        "steps_per_sweep": sweep,
         "kernel_ms": {n: round(t, 5) for n, t in ktimes.items()},
         "kernel_cells": kcells,
         "roofline": {"bound": "hbm", "kernel": dom_name,
-                     "basis": f"{b_step} B per fused cell-step + {b_half} B per other stress cell, "
-                              "per k_step launch",
+                     "basis": basis,
                      "algorithmic_bytes": int(dom_bytes),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
-        # the least k_step can move (68 B per fused cell-step, 152 with
-        # per-cell materials): its rate
+        # the least the dominant launch can move (68 B per fused cell, 152
+        # with per-cell materials, once per launch): its rate
         "k_step_min_bytes": {"bytes": int(min_bytes), "gbs": round(min_bytes / dom_s / 1e9, 1),
                              "frac": round(min_bytes / dom_s / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,

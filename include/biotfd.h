@@ -254,6 +254,30 @@ int bq_kernel_times(bq_handle *h, double out[3]);
  * blocks), out[1] by the stress half-step of k_step outside them, out[2] by
  * k_velocity. */
 int bq_kernel_cells(bq_handle *h, long long out[3]);
+
+/* Two steps per sweep (no counterpart in the reference, whose loop advances
+ * one step at a time, time_loop.F90:65-206). Where a grid is large enough, the
+ * interior cells at least 6 cells inside the interior-formula region (columns
+ * [9, MX-7), rows [9, MZ'-7)) are advanced two steps per pass over memory;
+ * the other cells one step at a time, as always. Results are bit-identical
+ * to single steps. bq_run uses it for each pair of steps after
+ * bq_set_sweep(h, 2) when the handle allows it: no learned boundary attached,
+ * and every receiver node outside that region or within 2 cells of its edge.
+ * The default, 1, keeps single steps (on MI355X they are faster: the
+ * two-step pass moves half the bytes per step but is bound by the latency of
+ * its longer chain per row). bq_get_sweep returns the steps per sweep
+ * bq_run will use (1 or 2). */
+int bq_set_sweep(bq_handle *h, int steps);
+int bq_get_sweep(bq_handle *h);
+/* Timing of the last timed bq_run by launch kind, ms per launch. With two
+ * steps per sweep (*steps_per_sweep = 2): ms[0] the two-step sweep of the
+ * region (two steps), ms[1] the stress half-step of the other cells with the
+ * samples, ms[2] their velocity half-step, ms[3] the edge rows (the last three
+ * once per step); cells[0..2] = region cells, other stress cells, other
+ * velocity cells. With single steps: ms[0..2] as bq_kernel_times, ms[3] = 0,
+ * cells[0..2] as bq_kernel_cells. cells[3] = 0. bq_kernel_times returns
+ * ms[0], ms[2], ms[3] of a two-step run. */
+int bq_kernel_stats(bq_handle *h, double ms[4], long long cells[4], int *steps_per_sweep);
 void bq_destroy(bq_handle *h);
 
 #ifdef __cplusplus

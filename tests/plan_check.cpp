@@ -73,6 +73,68 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
   return 0;
 }
 
+// The two-step partition: every cell of each half-step is either in the
+// region or covered once by the runs and the per-cell path; the region's
+// halos (level n+1 stresses 6 cells, velocities 4 cells around its own cells)
+// use only interior-formula cells; its reads (8 cells around) stay in the grid.
+static int check2(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) {
+  GridDesc g{};
+  g.MX = MX; g.MZ = MZ; g.T = T; g.PMLL = 1 - T; g.PMLXH = MX + T; g.PMLZH = MZ + T;
+  g.NX = MX + 2 * T; g.NZ = MZ + T + 1;
+  const int off = T - 1;
+  const Plan2 p = plan_cells2(g, off, fw, fr, nr, zs, zv);
+  const Fused2 &f = p.f2;
+  if (f.nfx == 0) return 0;
+  const int NX = g.NX, NZ = g.NZ;
+  std::vector<int> cs((size_t)NX * NZ, 0), cv((size_t)NX * NZ, 0);
+  auto at = [&](std::vector<int> &v, int x, int z) -> int & { return v[(size_t)z * NX + (x - g.PMLL)]; };
+  auto inside = [&](int x, int z) { return x >= g.PMLL && x <= g.PMLXH && z >= 0 && z <= g.PMLZH; };
+  if ((f.x0 + off) % 32 != 0) return fail("region alignment", MX, MZ, T, f.x0, f.z0);
+  if (f.x1 != f.x0 + fw * f.nfx || f.nfz != (f.z1 - f.z0 + fr - 1) / fr)
+    return fail("region tiles", MX, MZ, T, f.x0, f.z0);
+  // halos: stresses of level n+1 at [x0-6, x1+5] x [z0-6, z1+5] (interior
+  // stress formulas: columns [3, MX-2], rows [3, MZ'-2]); velocities at
+  // [x0-4, x1+3] x [z0-4, z1+3] (columns [4, MX-2], rows [2, MZ'-2])
+  if (f.x0 - 6 < 3 || f.x1 + 5 > MX - 2 || f.z0 - 6 < 3 || f.z1 + 5 > MZ - 2)
+    return fail("region stress halo", MX, MZ, T, f.x0, f.z0);
+  if (f.x0 - 4 < 4 || f.x1 + 3 > MX - 2 || f.z0 - 4 < 2 || f.z1 + 3 > MZ - 2)
+    return fail("region velocity halo", MX, MZ, T, f.x0, f.z0);
+  // reads: velocities 8 cells around (rows to z1+6 for W and QZ of the last row)
+  if (!inside(f.x0 - 8, f.z0 - 8) || !inside(f.x1 + 7, f.z1 + 7)) return fail("region reads", MX, MZ, T, f.x0, f.z0);
+  for (int z = f.z0; z < f.z1; ++z)
+    for (int x = f.x0; x < f.x1; ++x) { ++at(cs, x, z); ++at(cv, x, z); }
+  long long nr_s = 0, nr_v = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const Split &s = pass ? p.sv : p.ss;
+    std::vector<int> &cnt = pass ? cv : cs;
+    long long &n = pass ? nr_v : nr_s;
+    const int zmin = pass ? 2 : 5;
+    for (int q = 0; q < s.nrect; ++q) {
+      const int x0 = s.qx0[q], x1 = x0 + 64 * s.qntx[q];
+      if ((x0 + off) % 32 != 0) return fail("run alignment", MX, MZ, T, x0, s.qz0[q]);
+      if (x0 < 4 || x1 > MX - 1 || s.qz0[q] < zmin || s.qz1[q] > MZ - 1)
+        return fail("run range", MX, MZ, T, x0, s.qz0[q]);
+      for (int z = s.qz0[q]; z < s.qz1[q]; ++z)
+        for (int x = x0; x < x1; ++x) { ++at(cnt, x, z); ++n; }
+    }
+    for (int r = 0; r < kMaxCells; ++r)
+      for (int z = s.rz0[r]; z < s.rz1[r]; ++z)
+        for (int x = s.rx0[r]; x < s.rx1[r]; ++x) {
+          if (!inside(x, z)) return fail("cell outside grid", MX, MZ, T, x, z);
+          ++at(cnt, x, z);
+          ++n;
+        }
+  }
+  if (nr_s != p.stress_cells || nr_v != p.velocity_cells) return fail("cell counts 2", MX, MZ, T, 0, 0);
+  for (int z = 0; z <= g.PMLZH; ++z)
+    for (int x = g.PMLL; x <= g.PMLXH; ++x) {
+      const int want_s = z >= zs ? 1 : 0, want_v = (z >= zv && z <= g.PMLZH - 1) ? 1 : 0;
+      if (at(cs, x, z) != want_s) return fail("stress coverage 2", MX, MZ, T, x, z);
+      if (at(cv, x, z) != want_v) return fail("velocity coverage 2", MX, MZ, T, x, z);
+    }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 9 || (argc - 6) % 3 != 0) {
     std::printf("usage: plan_check FW FR NR ZS ZV MX MZ T [MX MZ T ...]\n");
@@ -83,6 +145,9 @@ int main(int argc, char **argv) {
   int n = 0;
   for (int a = 6; a + 2 < argc; a += 3) {
     if (check(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), fw, fr, nr, zs, zv)) return 1;
+    for (int w2 : {256, 192})
+      if (check2(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), w2, fr, nr, zs, zv))
+        return 1;
     ++n;
   }
   std::printf("ok %d\n", n);

@@ -5,8 +5,9 @@
   reference writes, by SHA-256 digest (tests/golden/cfg0_digests.npz, written
   by the reference itself).
 * The layered half-space (4000 x 2000) and the roofline model (8000 x 4000,
-  with quantized materials and with one material per cell, JMNUM = 32.2 M)
-  on their full grids for a bounded number of steps, bit for bit against the
+  with quantized materials, with one material per cell, JMNUM = 32.0 M, and
+  with two steps per sweep) on their full grids for a bounded number of
+  steps, bit for bit against the
   binary32 oracle on the host: from random initial fields, so that every cell
   of every row class - side and bottom PML strips, their memories, the
   surface rows, the bottom row and its history, every fused tile - is active
@@ -104,6 +105,7 @@ def _cfg2_job(steps, per_cell=False):
     ("layered 4000x2000", 24, "random"), ("layered 4000x2000", 300, "rest"),
     ("roofline 8000x4000", 12, "random"), ("roofline 8000x4000", 120, "rest"),
     ("roofline 8000x4000 one material per cell", 6, "random"),
+    ("roofline 8000x4000 two steps per sweep", 13, "random"),
 ])
 def test_full_grid_bitexact_vs_oracle(which, steps, start):
     t0 = time.perf_counter()
@@ -113,6 +115,9 @@ def test_full_grid_bitexact_vs_oracle(which, steps, start):
     print(f"{which} ({start}): {nx}x{nz} cells, JMNUM={job.table.shape[0]}, {steps} steps, "
           f"{len(job.receivers)} receivers", flush=True)
     sim = lib.Simulation(job)
+    if "two steps" in which:
+        sim.set_sweep(2)
+        assert sim.sweep() == 2
     ref = oracle.OracleSim(job, "f32")
     if start == "random":
         _random_start(job, sim, ref, seed=7)
