@@ -8,6 +8,8 @@ over all steps) on long runs, printing one JSON line per case:
 * the same model statistics on a 1000 x 500 grid for its full 8000 steps
   (--steps-long), so that the error growth over the workload's whole record
   is stated at a size the binary64 oracle finishes in minutes;
+* optionally the 4000 x 2000 layered half-space on its full grid
+  (--steps-layered, 5000 for its whole record);
 * the learned-boundary workload's judge (40-cell PML, binary32 on the GPU)
   and the runs it judges, against the binary64 oracle's 40-cell PML run, on
   the smallest held-out shots (--judge-shots).
@@ -91,7 +93,8 @@ def judge(n_shots, steps):
         ref.run(steps)
         tref = ref.traces()
         ref.close()
-        runs = {"gpu_fp32_pml40": lib.Simulation(jobs[40]), "learned_strip8": lib.Simulation(jobs[8]),
+        judge = lib.Simulation(jobs[40])
+        runs = {"gpu_fp32_pml40": judge, "learned_strip8": net.strip_grid(jobs[8], judge),
                 "plain_pml8": lib.Simulation(jobs[8]), "plain_pml20": lib.Simulation(jobs[20])}
         net.attach(runs["learned_strip8"])
         out = {"case": f"learned-boundary judge, held-out shot {i}", "grid_pml40": list(hostsetup.grid_shape(jobs[40]))[::-1],
@@ -111,7 +114,13 @@ def main():
     ap.add_argument("--steps-full", type=int, default=600)
     ap.add_argument("--steps-long", type=int, default=8000)
     ap.add_argument("--judge-shots", type=int, default=2)
+    ap.add_argument("--steps-layered", type=int, default=0,
+                    help="also the 4000 x 2000 layered half-space for this many steps (5000: its record)")
     args = ap.parse_args()
+    if args.steps_layered:
+        state("layered half-space, full 4000x2000 grid",
+              models.layered_job(name="LAYER", mx=4000, mz=2000, h=2.0, steps=args.steps_layered,
+                                 n_layers=10, seed=1, tpml=20, f0=25.0, n_rec=1000))
     if args.judge_shots:
         judge(args.judge_shots, 3000)
     if args.steps_long:
