@@ -23,6 +23,7 @@ import time
 import numpy as np
 
 from . import lib, models
+from .formats import Job
 from .lib import BiotError, Simulation, _ptr
 
 S = 8                 # strip cells
@@ -142,14 +143,29 @@ class Network:
     output gain; on the device as a ``bq_lb`` for :meth:`attach`."""
 
     def __init__(self, weights: np.ndarray, scales: np.ndarray, residual: bool = True,
-                 gain: float = 1.0, inner: int = S, damping: bool = False):
+                 gain: float = 1.0, inner: int = S, damping: bool = False,
+                 profile: str = "cpml8"):
         self.weights = np.ascontiguousarray(weights, np.float32).reshape(3, NSIDE)
         self.scales = np.ascontiguousarray(scales, np.float32).reshape(3, F)
         self.residual = bool(residual)
         self.damping = bool(damping) and self.residual
         self.gain = float(gain)
         self.inner = int(inner)
+        if profile not in PROFILES:
+            raise ValueError(f"profile must be one of {PROFILES}")
+        self.profile = profile
         self._h = None
+
+    def strip_grid(self, job8: Job, wide: Simulation | None = None) -> Simulation:
+        """A Simulation of job8 (TPML = 8) with the PML this network was
+        trained on: the 8-cell CPML ("cpml8"), or the inner 8 nodes of the
+        40-cell run's profile ("inner40", taken from wide, a TPML = 40
+        Simulation of the same model)."""
+        if self.profile == "cpml8":
+            return Simulation(job8)
+        if wide is None:
+            raise ValueError("the inner40 profile needs the 40-cell run of the same model")
+        return Simulation(job8, pml_arrays=inner_profile_pml(wide, job8.tpml))
 
     @property
     def form(self) -> int:
@@ -162,18 +178,19 @@ class Network:
         g = float(z["gain"]) if gain is None and "gain" in z else (1.0 if gain is None else gain)
         return cls(z["weights"], z["scales"], bool(z["residual"]) if "residual" in z else True, g,
                    int(z["inner"]) if "inner" in z else S,
-                   bool(z["damping"]) if "damping" in z else False)
+                   bool(z["damping"]) if "damping" in z else False,
+                   str(z["profile"]) if "profile" in z else "cpml8")
 
     def with_gain(self, gain: float, damping: bool | None = None) -> "Network":
         return Network(self.weights, self.scales, self.residual, gain, self.inner,
-                       self.damping if damping is None else damping)
+                       self.damping if damping is None else damping, self.profile)
 
     def save(self, path: str, **info):
         os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
         np.savez_compressed(path, weights=self.weights, scales=self.scales,
                             residual=np.asarray(self.residual), damping=np.asarray(self.damping),
                             gain=np.asarray(self.gain),
-                            inner=np.asarray(self.inner),
+                            inner=np.asarray(self.inner), profile=np.asarray(self.profile),
                             **{k: np.asarray(v) for k, v in info.items()})
 
     def device(self):
@@ -209,6 +226,37 @@ class Network:
             self.close()
         except Exception:
             pass
+
+
+PROFILES = ("cpml8", "inner40")
+
+
+def inner_profile_pml(wide: Simulation, tpml: int = S) -> list:
+    """The 24 PML coefficient arrays (bq_upload_pml order) of a grid with TPML
+    = tpml cut from those of wide, a run of the same model with a wider PML:
+    every strip cell gets the coefficients the wide run's cell at the same
+    place has, so a truncated grid whose strip holds the wide run's values
+    advances every cell at least 3 from its outer edge exactly as the wide run
+    does. Layouts: x arrays [row][node] over the rows, z arrays [column][node]
+    over the columns (nodes 0..T or 0..T+1)."""
+    tw = wide.job.tpml
+    if tw < tpml:
+        raise ValueError("the wide run's PML must be at least as wide")
+    nzw, nxw = wide.nz, wide.nx
+    nz, nx = wide.job.mzp + tpml + 1, wide.job.mx + 2 * tpml
+    out = []
+    for k in range(24):
+        a = wide.pml(k)
+        n = a.size
+        if n == (tw + 1) * nzw:
+            out.append(np.ascontiguousarray(a.reshape(nzw, tw + 1)[:nz, :tpml + 1]))
+        elif n == (tw + 2) * nzw:
+            out.append(np.ascontiguousarray(a.reshape(nzw, tw + 2)[:nz, :tpml + 2]))
+        elif n == (tw + 1) * nxw:
+            out.append(np.ascontiguousarray(a.reshape(nxw, tw + 1)[tw - tpml:tw - tpml + nx, :tpml + 1]))
+        else:
+            out.append(np.ascontiguousarray(a.reshape(nxw, tw + 2)[tw - tpml:tw - tpml + nx, :tpml + 2]))
+    return out
 
 
 def band_host(fields: dict, mx: int, mzp: int, tpml: int) -> np.ndarray:
@@ -321,7 +369,7 @@ def training_job(seed: int, steps: int, tpml: int = 40, mx: int = 1000, mz: int 
 
 
 def record(trainer: Trainer, n_models: int, steps: int, first: int, every: int,
-           per_batch: int = 50, log=print) -> int:
+           per_batch: int = 50, log=print, profile: str = "cpml8") -> int:
     """Runs n_models wide-PML (TPML = 40) training models, per_batch at a time
     as one batch, and records one group per model at t = first, first + every,
     ... < steps - 1: levels t-3 .. t of the wide run as input and the target
@@ -329,7 +377,9 @@ def record(trainer: Trainer, n_models: int, steps: int, first: int, every: int,
     strip is copied from the wide run after every step, so that it stays on
     the wide run's trajectory; level 4 is the wide band at t+1 minus that
     grid's band after its own step t+1 (before the copy). Direct trainer:
-    level 4 is the wide band at t+1. Returns the number of groups."""
+    level 4 is the wide band at t+1. profile: the truncated grid's PML, the
+    8-cell CPML or the inner 8 nodes of the 40-cell one (inner_profile_pml).
+    Returns the number of groups."""
     times = list(range(first, steps - 1, every))
     # steps at which each level is taken: t-3, t-2, t-1, t (inputs), t+1
     when = {}
@@ -345,7 +395,11 @@ def record(trainer: Trainer, n_models: int, steps: int, first: int, every: int,
         bw = lib.Batch(wide)
         trunc = bt = None
         if trainer.residual:
-            trunc = [Simulation(training_job(s, steps, 8)) for s in seeds]
+            if profile == "inner40":
+                trunc = [Simulation(training_job(s, steps, 8), pml_arrays=inner_profile_pml(w))
+                         for s, w in zip(seeds, wide)]
+            else:
+                trunc = [Simulation(training_job(s, steps, 8)) for s in seeds]
             bt = lib.Batch(trunc)
         t0 = time.perf_counter()
         last = max(when)

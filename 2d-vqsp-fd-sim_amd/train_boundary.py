@@ -41,17 +41,20 @@ def log(msg):
 
 def check(net, gains, n, steps, every, size=1000, forms=(False,)):
     """Reflected energy of the learned boundary at each gain (and form: damping
-    False/True) and of the plain 8-cell boundary against the 40-cell PML run
-    on n held-out models."""
+    False/True), of the plain 8-cell boundary and of the network's strip
+    without the network against the 40-cell PML run on n held-out models."""
     out = []
     for k in range(n):
         seed = 10_000 + k
         wide = lib.Simulation(learned.training_job(seed, steps, 40, size, size))
-        runs = {"plain8": lib.Simulation(learned.training_job(seed, steps, 8, size, size))}
+        job8 = lambda: learned.training_job(seed, steps, 8, size, size)
+        runs = {"plain8": lib.Simulation(job8())}
+        if net.profile != "cpml8":
+            runs[f"{net.profile} alone"] = net.strip_grid(job8(), wide)
         nets = []
         for damp in forms:
             for g in gains:
-                s = lib.Simulation(learned.training_job(seed, steps, 8, size, size))
+                s = net.strip_grid(job8(), wide)
                 nets.append(net.with_gain(g, damping=damp))
                 nets[-1].attach(s)
                 runs[f"{'damping' if damp else 'learned'}@{g:g}"] = s
@@ -95,6 +98,9 @@ def main():
     ap.add_argument("--damping", action="store_true",
                     help="also check the damping form of the residual network")
     ap.add_argument("--load", default=None, help="check this saved network only")
+    ap.add_argument("--profile", choices=learned.PROFILES, default="cpml8",
+                    help="the 8-cell strip's PML: the 8-cell CPML or the inner 8 nodes of the "
+                         "40-cell profile")
     args = ap.parse_args()
     gains = [float(g) for g in args.gains.split(",")]
     forms = (False, True) if args.damping else (False,)
@@ -107,18 +113,20 @@ def main():
     per_model = len(range(args.first, args.steps - 1, args.every))
     tr = learned.Trainer(1000, 1002, max_groups=args.models * per_model, batch=args.batch,
                          residual=not args.direct, inner=args.inner)
-    ng = learned.record(tr, args.models, args.steps, args.first, args.every, log=log)
+    ng = learned.record(tr, args.models, args.steps, args.first, args.every, log=log,
+                        profile=args.profile)
     t_rec = time.perf_counter() - t0
     scales = tr.compute_scales(ng)
     log(f"[lb] {ng} groups in {t_rec:.1f} s; scales {np.array2string(scales, precision=3)}")
     t1 = time.perf_counter()
     hist = learned.train(tr, learned.init_weights(args.seed), args.iters, args.lr, seed=args.seed, log=log)
     t_train = time.perf_counter() - t1
-    net = learned.Network(tr.get_weights(), scales, residual=not args.direct, inner=args.inner)
+    net = learned.Network(tr.get_weights(), scales, residual=not args.direct, inner=args.inner,
+                          profile=args.profile)
     res = check(net, gains, args.check, args.steps, 500, forms=forms) if args.check else []
     summary = {"groups": ng, "record_s": round(t_rec, 1), "train_s": round(t_train, 1),
                "iters": args.iters, "lr": args.lr, "residual": not args.direct,
-               "inner": args.inner,
+               "inner": args.inner, "profile": args.profile,
                "final_train_loss": [h[-1] for h in hist["train"]],
                "final_val_loss": [h[-1] for h in hist["val"]],
                "zero_output_val_loss": hist["zero_val"],

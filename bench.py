@@ -228,7 +228,9 @@ def run_learned(args):
     cells = {t: [int(np.prod(hostsetup.grid_shape(j))) for j in js] for t, js in jobs.items()}
     log(f"[bench] {n} held-out shots; cells per step: " +
         ", ".join(f"TPML {t}: {sum(c) / 1e6:.1f} M" for t, c in cells.items()))
-    sims = {k: [lib.Simulation(j) for j in jobs[t]] for k, t in (("wide", 40), ("learned", 8), ("plain8", 8))}
+    sims = {k: [lib.Simulation(j) for j in jobs[t]] for k, t in (("wide", 40), ("plain8", 8))}
+    # the network's strip: the 8-cell CPML or the inner nodes of the judge's profile
+    sims["learned"] = [net.strip_grid(j, w) for j, w in zip(jobs[8], sims["wide"])]
     for s in sims["learned"]:
         net.attach(s)
     batches = {k: lib.Batch(v) for k, v in sims.items()}
@@ -324,7 +326,7 @@ def run_learned(args):
                                f"dx=2 m, learned 8-cell boundary + batching; judge: 40-cell PML",
                    "shots": n, "cells_per_step_strip8": sum(cells[8]),
                    "network": {"residual": net.residual, "damping": net.damping, "gain": net.gain,
-                               "inner": net.inner}},
+                               "inner": net.inner, "profile": net.profile}},
         "value_basis": "all shots delivered at <= 1e-3 trace misfit: the learned batch plus the "
                        "fallback batch of the rejected shots",
         "plain_gpu_form_shots_per_s": round(n / t_plain, 3),

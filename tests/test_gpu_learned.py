@@ -225,3 +225,26 @@ def test_strip_copy_and_residual_capture():
         want = float((d.astype(np.float64) ** 2).mean())
         got = tr.step(s, np.array([0, 1], np.int32), 0.0)
         assert want > 0 and abs(got - want) <= 1e-5 * want, (s, got, want)
+
+
+def test_inner_profile_strip_tracks_the_wide_run():
+    """An 8-cell grid with the inner 8 nodes of a 40-cell PML's coefficients
+    (learned.inner_profile_pml), started from the wide run's fields at the same
+    cells, advances one step exactly as the wide run does at every cell at
+    least 3 from its outer edge (the others miss the wide run's outer cells)."""
+    wide_job = jobs.tiny_layered(steps=10, tpml=40)
+    job8 = jobs.tiny_layered(steps=10, tpml=8)
+    wide = lib.Simulation(wide_job)
+    _random_state(wide, 21)
+    trunc = lib.Simulation(job8, pml_arrays=learned.inner_profile_pml(wide, 8))
+    off = 40 - 8
+    for f in FIELDS:
+        trunc.set_field(f, np.ascontiguousarray(wide.field(f)[:trunc.nz, off:off + trunc.nx]))
+    wide.run(1)
+    trunc.run(1)
+    for f in FIELDS:
+        a = trunc.field(f)[:trunc.nz - 3, 3:trunc.nx - 3]
+        b = wide.field(f)[:trunc.nz - 3, off + 3:off + trunc.nx - 3]
+        np.testing.assert_array_equal(a, b, err_msg=f)
+    wide.close()
+    trunc.close()
