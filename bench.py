@@ -84,6 +84,25 @@ def cpu_reference_baseline(job_full, steps_a=2, steps_b=12, budget_s=300.0):
     return per_step
 
 
+def cpu_port_baseline(job, steps=3):
+    """The CPU baseline on all host threads: the binary32 oracle (a C
+    restatement of the reference's serial loop, OpenMP over rows, built with
+    -O2 and no contraction) on the same grid; per step over `steps` steps
+    after one untimed step. Returns (Gcell/s, threads)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref = oracle.OracleSim(job, "f32")
+    ref.run(1)
+    t0 = time.perf_counter()
+    ref.run(steps)
+    dt = (time.perf_counter() - t0) / steps
+    ref.close()
+    nz, nx = hostsetup.grid_shape(job)
+    print(f"[bench] oracle on {threads} threads: {dt:.2f} s per step", file=sys.stderr, flush=True)
+    return nx * nz / dt / 1e9, threads
+
+
 def make_like(job, steps):
     import copy
     j = copy.copy(job)
@@ -457,7 +476,7 @@ def main():
         copy_gbs = probes.probe_copy(1 << 28, 20)
         march_gbs, _ = probes.probe_march(nx, nz, 20)
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline and not per_cell:
         per_step = cpu_reference_baseline(job)
         if per_step and per_step > 0:
@@ -465,6 +484,13 @@ def main():
                    "kind": "reference",
                    "sample": f"reference fd2d (flang -O2 -ffp-contract=off, serial) on the same "
                              f"{nx}x{nz} grid, 12 minus 2 steps, no output"}
+        try:
+            v, th = cpu_port_baseline(job)
+            cpu_all = {"value": round(v, 5), "unit": "Gcell-updates/s", "cores": th, "kind": "port",
+                       "sample": f"binary32 oracle (C restatement of the serial loop, OpenMP over "
+                                 f"rows) on the same {nx}x{nz} grid, 3 steps after 1"}
+        except Exception as exc:  # the baseline is reported, never required
+            print(f"[bench] all-core baseline skipped: {exc}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "Gcell-updates/s & %HBM roofline",
@@ -495,6 +521,7 @@ def main():
         "k_step_min_bytes": {"bytes": int(min_bytes), "gbs": round(min_bytes / dom_s / 1e9, 1),
                              "frac": round(min_bytes / dom_s / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     if copy_gbs is not None:
         out["copy_probe_gbs"] = round(copy_gbs, 1)
