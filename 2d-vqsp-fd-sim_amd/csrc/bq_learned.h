@@ -54,7 +54,7 @@ constexpr int kLbH = 64;                // hidden channels = outputs (8 fields x
 constexpr int kLbTap = 5;               // taps of each convolution
 constexpr int kLbP = 52;                // valid output positions per inference block
 constexpr int kLbRows = 68;             // LDS rows of each activation tile
-constexpr int kLbXS = kLbCin + 8;       // LDS row stride of the input tile (bf16)
+constexpr int kLbXH = kLbCin / 2 + 8;   // LDS row stride of half the input tile (bf16)
 constexpr int kLbHS = kLbH + 8;         // LDS row stride of the hidden tiles (bf16)
 constexpr int kLbBlock = 256;
 enum { kLbDirect = 0, kLbResidual = 1, kLbDamping = 2 };
@@ -130,36 +130,49 @@ __device__ __forceinline__ const float *lb_level(const LbLevels &v, int lv) {
   return lv == 0 ? v.p0 : lv == 1 ? v.p1 : lv == 2 ? v.p2 : v.p3;
 }
 
+// The first layer's input is staged in two halves of its channels (levels
+// 0-1, then 2-3), so that three blocks fit in a CU's LDS.
 struct alignas(16) LbLds {
-  unsigned short x[kLbRows * kLbXS];
+  unsigned short x[kLbRows * kLbXH];
   unsigned short h1[kLbRows * kLbHS];
   unsigned short h2[kLbRows * kLbHS];
 };
 
 // One 5-tap convolution layer on the matrix cores for a tile of 64 output
 // rows: out[r][co] = sum_t sum_ci in[r + t][ci] * w[t][co][ci], in and out in
-// LDS (bf16, row stride is), w in global memory. Wave v computes the 16
-// outputs 16v..16v+15 for all 64 rows (four 16-row tiles), so each weight
-// fragment is loaded once per block and used four times. The MFMA 16x16x16
-// operand layout: lane q holds A[row q % 16][k 4(q/16) .. +3] and
-// B[k 4(q/16) .. +3][col q % 16]; the result holds D[row 4(q/16) + j][col
-// q % 16], j = 0..3.
-template <int CIN>
-__device__ __forceinline__ void lb_layer(const unsigned short *in, int is,
-                                         const unsigned short *__restrict__ w, lb_f32x4 acc[4]) {
+// LDS (bf16, row stride is), w in global memory in fragment order (below).
+// Wave v computes the 16 outputs 16v..16v+15 for all 64 rows (four 16-row
+// tiles), so each weight fragment is loaded once per block and used four
+// times. The MFMA 16x16x16 operand layout: lane q holds A[row q % 16][k
+// 4(q/16) .. +3] and B[k 4(q/16) .. +3][col q % 16]; the result holds D[row
+// 4(q/16) + j][col q % 16], j = 0..3.
+//
+// Fragment order: for each tap t, output block v (16 outputs) and input
+// block kc (16 inputs), the 64 lanes' B operands one after another - lane q's
+// four values W[t][16v + q%16][16kc + 4(q/16) .. +3] at (((t*4 + v) * CIN/16
+// + kc) * 64 + q) * 4 - so that a wave loads each fragment as 512
+// contiguous bytes.
+// The input blocks KC0 .. KC0+NK-1 of a layer of CIN inputs (in: those
+// NK*16 channels, row stride is), added to acc (zeroed first with ZERO).
+template <int CIN, int KC0, int NK, bool ZERO>
+__device__ __forceinline__ void lb_layer_part(const unsigned short *in, int is,
+                                              const unsigned short *__restrict__ w,
+                                              lb_f32x4 acc[4]) {
   const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
   const int qr = lane & 15, qk = 4 * (lane >> 4);
-  for (int m = 0; m < 4; ++m) acc[m] = lb_f32x4{0.f, 0.f, 0.f, 0.f};
+  if (ZERO)
+    for (int m = 0; m < 4; ++m) acc[m] = lb_f32x4{0.f, 0.f, 0.f, 0.f};
   for (int t = 0; t < kLbTap; ++t) {
-    const unsigned short *wrow = w + ((size_t)t * kLbH + 16 * v + qr) * CIN + qk;
+    const unsigned short *wfrag =
+        w + (((size_t)(t * 4 + v) * (CIN / 16) + KC0) * 64 + lane) * 4;
     const unsigned short *arow = in + (size_t)(qr + t) * is + qk;
     // every weight fragment of the tap in flight at once (one memory
     // latency per tap instead of one per few fragments)
-    lb_bf16x4 bw[CIN / 16];
+    lb_bf16x4 bw[NK];
 #pragma unroll
-    for (int k = 0; k < CIN / 16; ++k) bw[k] = *(const lb_bf16x4 *)(wrow + 16 * k);
+    for (int k = 0; k < NK; ++k) bw[k] = *(const lb_bf16x4 *)(wfrag + (size_t)k * 256);
 #pragma unroll
-    for (int cb = 0; cb < CIN; cb += 16) {
+    for (int cb = 0; cb < NK * 16; cb += 16) {
       const lb_bf16x4 b = bw[cb / 16];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
@@ -168,6 +181,12 @@ __device__ __forceinline__ void lb_layer(const unsigned short *in, int is,
       }
     }
   }
+}
+
+template <int CIN>
+__device__ __forceinline__ void lb_layer(const unsigned short *in, int is,
+                                         const unsigned short *__restrict__ w, lb_f32x4 acc[4]) {
+  lb_layer_part<CIN, 0, CIN / 16, true>(in, is, w, acc);
 }
 
 // Row and output channel of element j of acc[m] in lane `lane` of wave v.
@@ -201,35 +220,47 @@ __device__ __forceinline__ void lb_tile(const LbNet &net, int side, int L, size_
                                         const LbLevels lev, LbLds &S, Put put) {
   const float *inv = net.inv[side];
   // input rows r at positions p0 - 6 + r, as float4 runs of 4 cells of one
-  // field (12 cells per field): 24 per row and level
-  constexpr int kQ = kLbBand / 4, kN = kLbRows * kLbLev * kQ;
+  // field (12 cells per field): 24 per row and level; half h holds levels
+  // 2h, 2h+1
+  constexpr int kQ = kLbBand / 4, kN = kLbRows * 2 * kQ;
   constexpr int kIt = (kN + kLbBlock - 1) / kLbBlock;
-  float4 xs[kIt];
+  auto load_half = [&](int h, float4 *xs) {
 #pragma unroll
-  for (int k = 0; k < kIt; ++k) {   // all loads first
-    const int q = threadIdx.x + k * kLbBlock;
-    const int r = q / (kLbLev * kQ), rem = q % (kLbLev * kQ);
-    const int lv = rem / kQ, c4 = rem % kQ;
-    const int p = p0 - 6 + r;
-    xs[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (q < kN && p >= 0 && p < L)
-      xs[k] = *(const float4 *)(lb_level(lev, lv) + soff + (size_t)p * kLbBand + 4 * c4);
-  }
+    for (int k = 0; k < kIt; ++k) {
+      const int q = threadIdx.x + k * kLbBlock;
+      const int r = q / (2 * kQ), rem = q % (2 * kQ);
+      const int lv = 2 * h + rem / kQ, c4 = rem % kQ;
+      const int p = p0 - 6 + r;
+      xs[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (q < kN && p >= 0 && p < L)
+        xs[k] = *(const float4 *)(lb_level(lev, lv) + soff + (size_t)p * kLbBand + 4 * c4);
+    }
+  };
+  auto store_half = [&](const float4 *xs) {
 #pragma unroll
-  for (int k = 0; k < kIt; ++k) {
-    const int q = threadIdx.x + k * kLbBlock;
-    if (q >= kN) break;
-    const int r = q / (kLbLev * kQ), rem = q % (kLbLev * kQ);
-    const int lv = rem / kQ, c4 = rem % kQ;
-    const float4 x = xs[k];
-    const float sc = inv[(4 * c4) / kLbCells];
-    const unsigned int lo = lb_bf16(x.x * sc) | ((unsigned int)lb_bf16(x.y * sc) << 16);
-    const unsigned int hi = lb_bf16(x.z * sc) | ((unsigned int)lb_bf16(x.w * sc) << 16);
-    *(uint2 *)(S.x + r * kLbXS + lv * kLbBand + 4 * c4) = make_uint2(lo, hi);
-  }
+    for (int k = 0; k < kIt; ++k) {
+      const int q = threadIdx.x + k * kLbBlock;
+      if (q >= kN) break;
+      const int r = q / (2 * kQ), rem = q % (2 * kQ);
+      const int l2 = rem / kQ, c4 = rem % kQ;
+      const float4 x = xs[k];
+      const float sc = inv[(4 * c4) / kLbCells];
+      const unsigned int lo = lb_bf16(x.x * sc) | ((unsigned int)lb_bf16(x.y * sc) << 16);
+      const unsigned int hi = lb_bf16(x.z * sc) | ((unsigned int)lb_bf16(x.w * sc) << 16);
+      *(uint2 *)(S.x + r * kLbXH + l2 * kLbBand + 4 * c4) = make_uint2(lo, hi);
+    }
+  };
+  float4 xa[kIt];
+  load_half(0, xa);
+  store_half(xa);
   __syncthreads();
   lb_f32x4 acc[4];
-  lb_layer<kLbCin>(S.x, kLbXS, net.w1[side], acc);
+  lb_layer_part<kLbCin, 0, kLbCin / 32, true>(S.x, kLbXH, net.w1[side], acc);
+  load_half(1, xa);
+  __syncthreads();
+  store_half(xa);
+  __syncthreads();
+  lb_layer_part<kLbCin, kLbCin / 32, kLbCin / 32, false>(S.x, kLbXH, net.w1[side], acc);
   lb_store_hidden(S.h1, acc, p0 - 4, L);
   __syncthreads();
   lb_layer<kLbH>(S.h1, kLbHS, net.w2[side], acc);

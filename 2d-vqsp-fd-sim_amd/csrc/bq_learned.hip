@@ -392,18 +392,25 @@ int bq_lb_create(const float *weights, const float *scales, int form, float gain
   bq_lb *lb = new bq_lb();
   (void)hipGetDevice(&lb->device);
   std::vector<unsigned short> h((size_t)3 * kNSide);
-  // [tap][in][out] FP32 -> [tap][out][in] BF16
+  // [tap][in][out] FP32 -> BF16 in the fragment order of lb_layer: per tap,
+  // output block v, input block kc, lane q the four values
+  // W[t][16v + q%16][16kc + 4(q/16) .. +3]
   for (int s = 0; s < 3; ++s) {
     const float *src = weights + (size_t)s * kNSide;
     unsigned short *dst = h.data() + (size_t)s * kNSide;
     const int cins[3] = {kLbCin, kLbH, kLbH};
     size_t o = 0;
     for (int layer = 0; layer < 3; ++layer) {
-      const int cin = cins[layer];
+      const int cin = cins[layer], nk = cin / 16;
       for (int t = 0; t < kLbTap; ++t)
-        for (int co = 0; co < kLbH; ++co)
-          for (int ci = 0; ci < cin; ++ci)
-            dst[o + ((size_t)t * kLbH + co) * cin + ci] = host_bf16(src[o + ((size_t)t * cin + ci) * kLbH + co]);
+        for (int v = 0; v < kLbH / 16; ++v)
+          for (int kc = 0; kc < nk; ++kc)
+            for (int q = 0; q < 64; ++q)
+              for (int j = 0; j < 4; ++j) {
+                const int co = 16 * v + (q & 15), ci = 16 * kc + 4 * (q >> 4) + j;
+                const size_t at = ((((size_t)t * 4 + v) * nk + kc) * 64 + q) * 4 + j;
+                dst[o + at] = host_bf16(src[o + ((size_t)t * cin + ci) * kLbH + co]);
+              }
       o += (size_t)kLbTap * cin * kLbH;
     }
   }
