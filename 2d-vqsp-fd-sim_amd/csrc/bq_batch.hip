@@ -77,6 +77,8 @@ __device__ __forceinline__ int find_shot(const int *cum, int n, int b) {
   return lo;
 }
 
+__device__ __forceinline__ void other_item(const BatchArgs &B, int w, int kofs);
+
 // Item w of kb_step: items [0, total[C_FUSED]) are fused tiles (w is the tile
 // number, tiles numbered shot after shot, row-major within a shot); the rest
 // are the other cells' blocks and the sample blocks of every shot.
@@ -93,7 +95,12 @@ __device__ __forceinline__ void step_item(const BatchArgs &B, int w, int kofs, F
     return;
   }
   if (threadIdx.x >= kBlock) return;
-  w -= nf;
+  other_item(B, w - nf, kofs);
+}
+
+// Item w of the other cells' blocks and the sample blocks of every shot
+// (kBlock threads).
+__device__ __forceinline__ void other_item(const BatchArgs &B, int w, int kofs) {
   const int s = find_shot(B.cum + C_OTHER * (B.nshots + 1), B.nshots, w);
   const ShotDev &sd = B.shots[s];
   const KParams k = sd.kp[(sd.base_par + kofs) & 1];
@@ -186,6 +193,13 @@ __global__ __launch_bounds__(kStepBlock, 4) void kb_step(BatchArgs B, int kl) {
     step_item<FR>(B, w, kofs, L);
   }
   if (threadIdx.x == 0) persistent_done(B.queue);
+}
+
+// The other cells' stress blocks and the samples of every shot, without the
+// fused tiles' LDS, so that many of these short blocks share a CU (grid
+// schedule; kb_step then runs the fused tiles alone).
+__global__ __launch_bounds__(kBlock) void kb_rest(BatchArgs B, int kl) {
+  other_item(B, blockIdx.x, *B.dstep + kl);
 }
 
 template <bool PERSIST>
@@ -405,7 +419,11 @@ void launch_step_fr(bq_batch *b, const BatchArgs &a, int kl, int advance) {
       hipLaunchKernelGGL((kb_velocity<true>), dim3(std::min(nvel, kPersistVel * b->cus)),
                          dim3(kBlock), 0, b->stream, a, kl);
   } else {
-    if (nstep) hipLaunchKernelGGL((kb_step<FR, false>), dim3(nstep), dim3(kStepBlock), 0, b->stream, a, kl);
+    if (a.total[C_FUSED])
+      hipLaunchKernelGGL((kb_step<FR, false>), dim3(a.total[C_FUSED]), dim3(kStepBlock), 0, b->stream,
+                         a, kl);
+    if (a.total[C_OTHER])
+      hipLaunchKernelGGL(kb_rest, dim3(a.total[C_OTHER]), dim3(kBlock), 0, b->stream, a, kl);
     if (nvel) hipLaunchKernelGGL((kb_velocity<false>), dim3(nvel), dim3(kBlock), 0, b->stream, a, kl);
   }
   hipLaunchKernelGGL(kb_edges, dim3(a.total[C_EDGE]), dim3(kEdgeBlock), 0, b->stream, a, kl);

@@ -309,7 +309,18 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   // k_edges', row 0 of the velocities is never updated; without it every row
   // from 0 is a per-cell row
   const int z0 = k.SI ? 2 : 0;
-  const bq::Plan plan = bq::plan_cells(g, k.OFF, kFW, kFR, kRun, z0, z0);
+  // rows per fused block: the tallest of 128, 64, 32, 16 that still gives
+  // about two blocks per CU. A block marches its rows one after another, so
+  // on a grid with few tiles shorter blocks finish sooner; on a large grid
+  // taller ones recompute fewer halo rows.
+  bq::Plan plan{};
+  int fr = 16;
+  for (int c : {128, 64, 32, 16}) {
+    plan = bq::plan_cells(g, k.OFF, kFW, c, kRun, z0, z0);
+    fr = c;
+    if (plan.fu.nfx == 0 || plan.fu.nfx * plan.fu.nfz >= kFusedTarget) break;
+  }
+  h->fr = fr;
   h->fu = plan.fu;
   h->ss = plan.ss;
   h->sv = plan.sv;
@@ -632,9 +643,17 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
       ln.sample_step = it - 1;
       const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
-      if (g1)
-        hipLaunchKernelGGL((k_step<kRun, kFR>), dim3(g1), dim3(kStepBlock), 0, h->stream, k,
-                           h->fu, h->ss, ln, it);
+      if (g1) {
+        auto launch = [&](auto kern) {
+          hipLaunchKernelGGL(kern, dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu, h->ss, ln, it);
+        };
+        switch (h->fr) {
+          case 16: launch(k_step<kRun, 16>); break;
+          case 32: launch(k_step<kRun, 32>); break;
+          case 64: launch(k_step<kRun, 64>); break;
+          default: launch(k_step<kRun, 128>); break;
+        }
+      }
       if (e) (void)hipEventRecord(e[1], h->stream);
       if (vgrid)
         hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);

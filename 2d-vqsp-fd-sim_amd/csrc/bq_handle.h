@@ -37,6 +37,7 @@ struct bq_handle {
   int timed_steps = 0;
   Split ss{}, sv{};
   Fused fu{};
+  int fr = 128;  // rows per fused block of the single-grid launches
   long long fused_cells = 0, stress_cells = 0, velocity_cells = 0;
   // learned absorbing boundary (bq_lb_attach): the network, and the band of
   // the last four time levels, level n in slot n % 4

@@ -206,7 +206,7 @@ def run_batched(args):
                      "frac": round(gcell * B_STEP / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline and not per_cell:
+    if not args.no_cpu_baseline:
         # the reference, serial, on the median-sized shot: per-step time from a
         # 12- and a 2-step run, times the steps of a shot
         med = jobs[int(np.argsort(cells)[len(cells) // 2])]
