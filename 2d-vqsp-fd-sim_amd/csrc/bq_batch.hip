@@ -117,7 +117,7 @@ __device__ __forceinline__ void other_item(const BatchArgs &B, int w, int kofs) 
     int tile;
     const Split &sp = sd.ss;
     const int r = run_rect(sp, xcd_tile(b, sd.nruns), &tile);
-    stress_run<kRun>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+    stress_run<kRun>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                      threadIdx.x & 63, itile);
     return;
   }
@@ -140,7 +140,7 @@ __device__ __forceinline__ void velocity_item(const BatchArgs &B, int w, int kof
   }
   int tile;
   const int r = run_rect(sp, xcd_tile(b - sd.vrest, sd.vruns), &tile);
-  velocity_run<kRun>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+  velocity_run<kRun>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                      threadIdx.x & 63);
 }
 
@@ -361,11 +361,10 @@ int layout(bq_batch *b, int fr) {
     const bq_handle *h = b->shots[s];
     ShotDev &d = b->host[s];
     d.fu = h->fu;
-    d.fu.nfz = d.fu.nfx > 0 ? (d.fu.vz1 - d.fu.vz0 + fr - 1) / fr : 0;
-    if (d.fu.nfz == 0) d.fu.nfx = 0;
+    bq::fused_rows(d.fu, fr);
     d.ss = h->ss;
     d.sv = h->sv;
-    d.nfused = d.fu.nfx * d.fu.nfz;
+    d.nfused = bq::fused_blocks(d.fu);
     d.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
     d.nruns = h->ss.qcum[kMaxRuns];
     d.nsample = (h->mr + kBlock - 1) / kBlock;

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nr
   }
   int tile;
   const int r = run_rect(sp, xcd_tile(blockIdx.x - nrest, gridDim.x - nrest), &tile);
-  velocity_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+  velocity_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                    threadIdx.x & 63);
 }
 
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kStepBlock) void k_step(KParams k, Fused fu, Split 
   if (b < ln.nruns) {
     int tile;
     const int r = run_rect(sp, xcd_tile(b, ln.nruns), &tile);
-    stress_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+    stress_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                    threadIdx.x & 63, itile);
     return;
   }
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_rest_stress(KParams k, Split sp, Lau
   if (b < ln.nruns) {
     int tile;
     const int r = run_rect(sp, xcd_tile(b, ln.nruns), &tile);
-    stress_run<NR>(k, sp.qx0[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+    stress_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                    threadIdx.x & 63, itile);
     return;
   }
@@ -318,7 +318,8 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   for (int c : {128, 64, 32, 16}) {
     plan = bq::plan_cells(g, k.OFF, kFW, c, kRun, z0, z0);
     fr = c;
-    if (plan.fu.nfx == 0 || plan.fu.nfx * plan.fu.nfz >= kFusedTarget) break;
+    const int nb = bq::fused_blocks(plan.fu);
+    if (nb == 0 || nb >= kFusedTarget) break;
   }
   h->fr = fr;
   h->fu = plan.fu;
@@ -417,9 +418,17 @@ static int upload_pml_host(bq_handle *h, const bq::PmlHost &p) {
     HIPCHK(h, hipMemcpyAsync((void *)k.x1[t], p.x1[t].data(), p.x1[t].size() * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync((void *)k.x2[t], p.x2[t].data(), p.x2[t].size() * 4, hipMemcpyHostToDevice, h->stream));
   }
-  for (int t = 0; t < 4; ++t) {
-    HIPCHK(h, hipMemcpyAsync((void *)k.z1[t], p.z1[t].data(), p.z1[t].size() * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync((void *)k.z2[t], p.z2[t].data(), p.z2[t].size() * 4, hipMemcpyHostToDevice, h->stream));
+  // bottom coefficients: (0:T[+1], NXc) on the host, [j][ix] on the device
+  const int NX = h->g.NX, T = h->g.T;
+  std::vector<float> zt[8];
+  for (int t = 0; t < 8; ++t) {
+    const std::vector<float> &src = t < 4 ? p.z1[t] : p.z2[t - 4];
+    const int nj = t < 4 ? T + 1 : T + 2;
+    zt[t].resize(src.size());
+    for (int ix = 0; ix < NX; ++ix)
+      for (int j = 0; j < nj; ++j) zt[t][(size_t)j * NX + ix] = src[(size_t)ix * nj + j];
+    const float *dst = t < 4 ? k.z1[t] : k.z2[t - 4];
+    HIPCHK(h, hipMemcpyAsync((void *)dst, zt[t].data(), zt[t].size() * 4, hipMemcpyHostToDevice, h->stream));
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->pml_ok = true;
@@ -471,8 +480,17 @@ int bq_get_pml(bq_handle *h, int id, float *out) {
   const size_t n = grp == 0 ? (size_t)(g.T + 1) * g.NZ : grp == 1 ? (size_t)(g.T + 2) * g.NZ
                  : grp == 2 ? (size_t)(g.T + 1) * g.NX : (size_t)(g.T + 2) * g.NX;
   if (out) {
-    HIPCHK(h, hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (grp < 2) {
+      HIPCHK(h, hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+    } else {  // [j][ix] on the device -> (0:T[+1], NXc)
+      std::vector<float> t(n);
+      HIPCHK(h, hipMemcpyAsync(t.data(), src, n * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      const int nj = grp == 2 ? g.T + 1 : g.T + 2;
+      for (int ix = 0; ix < g.NX; ++ix)
+        for (int j = 0; j < nj; ++j) out[(size_t)ix * nj + j] = t[(size_t)j * g.NX + ix];
+    }
   }
   return (int)n;
 }
@@ -591,7 +609,7 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   const GridDesc &g = h->g;
   const dim3 blk(kBlock);
   Launch1 ln{};
-  ln.nfused = h->fu.nfx * h->fu.nfz;
+  ln.nfused = bq::fused_blocks(h->fu);
   ln.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
   ln.nruns = h->ss.qcum[kMaxRuns];
   const int vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;

@@ -123,11 +123,11 @@ static Split make_split(const Rect *runs, int nruns, const Rect *cells, int ncel
   s.qcum[0] = 0;
   for (int q = 0; q < nruns; ++q) {
     const Rect &r = runs[q];
-    const int ntx = r.x1 > r.x0 ? (r.x1 - r.x0) / 64 : 0;
+    const int ntx = r.x1 > r.x0 ? (r.x1 - r.x0 + 63) / 64 : 0;
     const int nz = r.z1 > r.z0 ? (r.z1 - r.z0 + 4 * nr - 1) / (4 * nr) : 0;
     if (ntx * nz == 0) continue;
     const int k = s.nrect++;
-    s.qx0[k] = r.x0; s.qntx[k] = ntx; s.qz0[k] = r.z0; s.qz1[k] = r.z1;
+    s.qx0[k] = r.x0; s.qxl[k] = r.x1; s.qntx[k] = ntx; s.qz0[k] = r.z0; s.qz1[k] = r.z1;
     s.qblk[k] = ntx * nz;
     s.qcum[k + 1] = s.qcum[k] + s.qblk[k];
   }
@@ -144,36 +144,51 @@ static Split make_split(const Rect *runs, int nruns, const Rect *cells, int ncel
 
 static long long split_cells(const Split &s) {
   long long n = s.cum[kMaxCells];
-  for (int q = 0; q < s.nrect; ++q) n += 64LL * s.qntx[q] * (s.qz1[q] - s.qz0[q]);
+  for (int q = 0; q < s.nrect; ++q) n += (long long)(s.qxl[q] - s.qx0[q]) * (s.qz1[q] - s.qz0[q]);
   return n;
+}
+
+void fused_rows(Fused &fu, int fr) {
+  const int rows = fu.vz1 - fu.vz0;
+  auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
+  fu.nfz = (fu.nfx > 0 && rows > 0) ? cdiv(rows, fr) : 0;
+  fu.nb2 = (fu.h2 && rows > 0) ? cdiv(cdiv(rows, fr / 2), 2) : 0;
+  fu.nb4 = (fu.h4 && rows > 0) ? cdiv(cdiv(rows, fr / 4), 4) : 0;
 }
 
 // Interior columns start at the first column >= 5 whose address is 128-byte
 // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
-// Runs (interior formulas, 64-column strips) cover columns [fx, xe), xe <=
-// MX-1. Fused tiles (full width only) cover columns [fx, vxe), vxe <= MX-3,
-// and rows [5, MZ'-3): their new stresses reach two rows and columns further,
-// still inside [3, MX-2] x [3, MZ'-2], where both half-steps use the interior
-// formulas. The runs take the columns right of the fused tiles and the rows
-// just above and below them; every other cell - the PML strips, the surface
-// and bottom rows, the columns right of the last strip - goes to the per-cell
-// path. The runs read rows up to z1+2 and columns x0-2 .. x1+1; with
-// z1 <= MZ'-1, TPML >= 1 and 4 <= x0, x1 <= MX-1 these stay inside the grid.
+// Fused tiles cover columns [fx, vxe), vxe <= MX-3, and rows [5, MZ'-3): their
+// new stresses reach two rows and columns further, still inside [3, MX-2] x
+// [3, MZ'-2], where both half-steps use the interior formulas. Full tiles
+// (fw columns) come first, then a column of half tiles and one of quarter
+// tiles where the rest of [fx, MX-3) holds them. Runs (interior formulas,
+// 64-column strips, the last one masked) take the columns [vxe, MX-1) right
+// of the tiles and the rows just above and below them; every other cell - the
+// PML strips, the surface and bottom rows, the columns from MX-1 - goes to the
+// per-cell path. The runs read rows up to z1+2 and columns x0-2 .. x0+64*ntx+1,
+// the columns clamped to the grid's last column; with z1 <= MZ'-1, TPML >= 1,
+// 4 <= x0 and qxl <= MX-1 the values they use stay inside the grid.
 Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv) {
   Plan p{};
   const int PX1 = g.PMLXH + 1, ZS1 = g.PMLZH + 1, ZV1 = g.PMLZH;  // exclusive ends
   int fx = 5;
   while ((fx + off) % 32 != 0) ++fx;
   const int ntx = g.MX - 2 >= fx ? (g.MX - 2 - fx + 1) / 64 : 0;
-  const int xe = fx + 64 * ntx;
+  const int xr = g.MX - 1;  // runs end (exclusive)
   Fused &fu = p.fu;
   fu.vx0 = fx;
-  fu.nfx = g.MX - 3 > fx ? (g.MX - 3 - fx) / fw : 0;
   fu.vz0 = 5;
   fu.vz1 = g.MZ - 3;
-  fu.nfz = (fu.nfx > 0 && fu.vz1 > fu.vz0) ? (fu.vz1 - fu.vz0 + fr - 1) / fr : 0;
-  if (fu.nfz == 0) fu.nfx = 0;
-  fu.vxe = fx + fw * fu.nfx;
+  const bool rows = fu.vz1 > fu.vz0 && ntx > 0;
+  const int width = rows && g.MX - 3 > fx ? g.MX - 3 - fx : 0;
+  fu.nfx = width / fw;
+  int rest = width - fw * fu.nfx;
+  fu.h2 = rest >= fw / 2;
+  rest -= fu.h2 * (fw / 2);
+  fu.h4 = rest >= fw / 4;
+  fu.vxe = fx + fw * fu.nfx + (fw / 2) * fu.h2 + (fw / 4) * fu.h4;
+  fused_rows(fu, fr);
   if (ntx == 0) {  // everything on the per-cell path
     const Rect cs[1] = {{g.PMLL, PX1, zs, ZS1}};
     const Rect cv[1] = {{g.PMLL, PX1, zv, ZV1}};
@@ -181,15 +196,16 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
     p.sv = make_split(nullptr, 0, cv, 1, nr);
   } else {
     const int vxe = fu.vxe;  // = fx without fused tiles
-    p.fused_cells = (long long)(vxe - fx) * (fu.vz1 - fu.vz0) * (fu.nfx ? 1 : 0);
-    const int zf = fu.nfx ? fu.vz1 : 5;  // first row below the fused tiles
-    const Rect rs[2] = {{vxe, xe, 5, g.MZ - 1}, {fx, vxe, zf, g.MZ - 1}};
-    const Rect cs[4] = {{g.PMLL, fx, zs, ZS1}, {xe, PX1, zs, ZS1}, {fx, xe, zs, 5},
-                        {fx, xe, g.MZ - 1, ZS1}};
+    const bool any = vxe > fx;
+    p.fused_cells = any ? (long long)(vxe - fx) * (fu.vz1 - fu.vz0) : 0;
+    const int zf = any ? fu.vz1 : 5;  // first row below the fused tiles
+    const Rect rs[2] = {{vxe, xr, 5, g.MZ - 1}, {fx, vxe, zf, g.MZ - 1}};
+    const Rect cs[4] = {{g.PMLL, fx, zs, ZS1}, {xr, PX1, zs, ZS1}, {fx, xr, zs, 5},
+                        {fx, xr, g.MZ - 1, ZS1}};
     p.ss = make_split(rs, 2, cs, 4, nr);
-    const Rect rv[3] = {{vxe, xe, 2, g.MZ - 1}, {fx, vxe, 2, 5}, {fx, vxe, zf, g.MZ - 1}};
-    const Rect cv[4] = {{g.PMLL, fx, zv, ZV1}, {xe, PX1, zv, ZV1}, {fx, xe, zv, 2},
-                        {fx, xe, g.MZ - 1, ZV1}};
+    const Rect rv[3] = {{vxe, xr, 2, g.MZ - 1}, {fx, vxe, 2, 5}, {fx, vxe, zf, g.MZ - 1}};
+    const Rect cv[4] = {{g.PMLL, fx, zv, ZV1}, {xr, PX1, zv, ZV1}, {fx, xr, zv, 2},
+                        {fx, xr, g.MZ - 1, ZV1}};
     p.sv = make_split(rv, 3, cv, 4, nr);
   }
   p.stress_cells = split_cells(p.ss);

@@ -42,24 +42,39 @@ constexpr int kMaxRuns = 4;   // run rectangles of one half-step
 constexpr int kMaxCells = 8;  // per-cell rectangles of one half-step
 
 // The cells of one half-step outside the fused tiles. Runs: up to kMaxRuns
-// rectangles of 64-column strips [qx0, qx0 + 64*qntx) x [qz0, qz1), in blocks
-// of 64 columns x 4 runs of nr rows (qblk blocks, qcum cumulative). Per-cell
-// path: up to kMaxCells rectangles [rx0, rx1) x [rz0, rz1), one thread per
-// cell, cum the cumulative cell counts.
+// rectangles [qx0, qxl) x [qz0, qz1) of qntx 64-column strips from qx0 (the
+// last strip ends past qxl when the width is not a multiple of 64: its lanes
+// beyond qxl compute but do not store), in blocks of 64 columns x 4 runs of
+// nr rows (qblk blocks, qcum cumulative). Per-cell path: up to kMaxCells
+// rectangles [rx0, rx1) x [rz0, rz1), one thread per cell, cum the cumulative
+// cell counts.
 struct Split {
   int nr, nrect;
-  int qx0[kMaxRuns], qntx[kMaxRuns], qz0[kMaxRuns], qz1[kMaxRuns], qblk[kMaxRuns];
+  int qx0[kMaxRuns], qxl[kMaxRuns], qntx[kMaxRuns], qz0[kMaxRuns], qz1[kMaxRuns], qblk[kMaxRuns];
   int qcum[kMaxRuns + 1];
   int rx0[kMaxCells], rx1[kMaxCells], rz0[kMaxCells], rz1[kMaxCells];
   int cum[kMaxCells + 1];
 };
 
-// Fused tiles: columns [vx0, vxe) in nfx tiles of the tile width, rows
-// [vz0, vz1) in nfz blocks of the block height (the last one clipped).
-// nfx = 0: no fused tiles.
+// Fused tiles over columns [vx0, vxe) and rows [vz0, vz1):
+//   nfx columns of full tiles (the tile width fw) from vx0, in nfz rows of
+//   blocks of the block height fr (the last one clipped);
+//   then, where h2 = 1, one column of half tiles (fw/2 wide, fr/2 rows), two
+//   to a block (nb2 blocks), and where h4 = 1, one column of quarter tiles
+//   (fw/4 wide, fr/4 rows), four to a block (nb4 blocks). A block's tiles are
+//   consecutive in z; tiles past vz1 are empty.
+// nfx = h2 = h4 = 0: no fused tiles. The block counts depend on the block
+// height (fused_rows).
 struct Fused {
   int vx0, vxe, nfx, vz0, vz1, nfz;
+  int h2, h4, nb2, nb4;
 };
+
+// Sets nfz, nb2, nb4 of fu for block height fr.
+void fused_rows(Fused &fu, int fr);
+
+// Blocks of a fused launch: full, half and quarter tiles.
+inline int fused_blocks(const Fused &fu) { return fu.nfx * fu.nfz + fu.nb2 + fu.nb4; }
 
 struct Plan {
   Fused fu;

@@ -54,8 +54,11 @@ struct KParams {
 #define FA(a, i, l) (a)[(size_t)(l) * (size_t)k.P + (size_t)((i) + k.OFF)]
 #define XL1(t, j, l) (k.x1[t][(l) * (T + 1) + (j)])
 #define XL2(t, j, l) (k.x2[t][(l) * (T + 2) + (j)])
-#define ZH1(t, j, ix) (k.z1[t][(ix) * (T + 1) + (j)])
-#define ZH2(t, j, ix) (k.z2[t][(ix) * (T + 2) + (j)])
+// the bottom PML coefficients sit as [j][ix] on the device (the reference's
+// (0:T, NXc) arrays transposed), so that a wavefront's cells of one row read
+// consecutive addresses
+#define ZH1(t, j, ix) (k.z1[t][(size_t)(j) * k.NX + (ix)])
+#define ZH2(t, j, ix) (k.z2[t][(size_t)(j) * k.NX + (ix)])
 
 // One-sided surface stencils of the explicit free surface (binary32 quotients,
 // as the reference's constant expressions fold them).
@@ -472,17 +475,23 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
   return x * per + min(x, rem) + q;
 }
 
+// A run's cells [x0, xl) x [z0, z1) in ntx strips of 64 columns; lanes at
+// columns >= xl (the last strip of a width that is not a multiple of 64)
+// compute but do not store, and their loads are clamped to the grid's last
+// column (the lanes below xl never use those values).
 template <int NR>
-__device__ __forceinline__ void stress_run(const KParams &k, int x0, int ntx, int z0, int z1,
-                                           int tile, int wv, int lane, int itile) {
+__device__ __forceinline__ void stress_run(const KParams &k, int x0, int xl, int ntx, int z0,
+                                           int z1, int tile, int wv, int lane, int itile) {
   const int tx = tile % ntx, tz = tile / ntx;
   const int c = x0 + tx * 64;
   const int i = c + lane;
   const int l0 = z0 + (tz * 4 + wv) * NR;
   if (l0 >= z1) return;
+  const bool st = i < xl;
   const int P = k.P;
-  const int ix = i + k.OFF;
-  const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
+  const int ix = min(i + k.OFF, k.NX - 1);
+  const int hx = min((lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF,
+                     k.NX - 1);
   const float *__restrict__ U = k.U0;
   const float *__restrict__ W = k.W0;
   const float *__restrict__ QX = k.QX0;
@@ -529,6 +538,7 @@ __device__ __forceinline__ void stress_run(const KParams &k, int x0, int ntx, in
     float qzz = (C * (zp1 - z0));
     ezz = AH * (wp2 - wm1) + BC * ezz;
     qzz = AH * (zp2 - zm1) + BC * qzz;
+    if (!st) continue;
     nTXZ[r + ix] = TXZ[r + ix] + DT * (s0.x * exz);
     const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
     const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
@@ -547,16 +557,18 @@ __device__ __forceinline__ void stress_run(const KParams &k, int x0, int ntx, in
 }
 
 template <int NR>
-__device__ __forceinline__ void velocity_run(const KParams &k, int x0, int ntx, int z0, int z1,
-                                             int tile, int wv, int lane) {
+__device__ __forceinline__ void velocity_run(const KParams &k, int x0, int xl, int ntx, int z0,
+                                             int z1, int tile, int wv, int lane) {
   const int tx = tile % ntx, tz = tile / ntx;
   const int c = x0 + tx * 64;
   const int i = c + lane;
   const int l0 = z0 + (tz * 4 + wv) * NR;
   if (l0 >= z1) return;
+  const bool st = i < xl;
   const int P = k.P;
-  const int ix = i + k.OFF;
-  const int hx = (lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF;
+  const int ix = min(i + k.OFF, k.NX - 1);
+  const int hx = min((lane == 0 ? c + 64 : lane == 1 ? c + 65 : lane == 2 ? c - 1 : lane == 3 ? c - 2 : i) + k.OFF,
+                     k.NX - 1);
   const float *__restrict__ TXX = k.TXX1;
   const float *__restrict__ TZZ = k.TZZ1;
   const float *__restrict__ TXZ = k.TXZ1;
@@ -595,6 +607,7 @@ __device__ __forceinline__ void velocity_run(const KParams &k, int x0, int ntx, 
     const XNb pn = xneighbours(p0, PR[r + hx], lane);
     const int jm = JM[r + ix] - 1;
     const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
+    if (!st) continue;
     const float q = QX[r + ix], u = U[r + ix], qz = QZ[r + ix], w = W[r + ix];
     const float sx = A * (xx.p1 - xx.m2 + xz[j + 4] - xz[j + 1]) + B * (txx0 - xx.m1 + xz[j + 3] - xz0);
     const float px = A * (pn.p1 - pn.m2) + B * (p0 - pn.m1);
@@ -745,104 +758,80 @@ __device__ __forceinline__ void ring_velocity(const KParams &k, const float *X, 
                   qz, w, v0, v1, v2, o);
 }
 
-__device__ __forceinline__ void fused_stress(const KParams &k, const float (*sU)[kLW],
-                                             const float (*sW)[kLW], const float (*sZ)[kLW],
-                                             const float (*sQ)[kLW], int s, int q, int i,
-                                             float txx, float tzz, float txz, float pr, int jm,
-                                             int itile, float *o) {
-  const int u0s = s % 5, w0s = s % 6, qs = s & 3;
-  const float um2 = sU[(s - 2) % 5][q], um1 = sU[(s - 1) % 5][q], u0 = sU[u0s][q];
-  const float up1 = sU[(s + 1) % 5][q];
-  const float uxm1 = sU[u0s][q - 1], uxp1 = sU[u0s][q + 1], uxp2 = sU[u0s][q + 2];
-  const float wm1 = sW[(s - 1) % 6][q], w0 = sW[w0s][q], wp1 = sW[(s + 1) % 6][q];
-  const float wp2 = sW[(s + 2) % 6][q];
-  const float wxm2 = sW[w0s][q - 2], wxm1 = sW[w0s][q - 1], wxp1 = sW[w0s][q + 1];
-  const float zm1 = sZ[(s - 1) % 6][q], z0 = sZ[w0s][q], zp1 = sZ[(s + 1) % 6][q];
-  const float zp2 = sZ[(s + 2) % 6][q];
-  const float qxm1 = sQ[qs][q - 1], qx0 = sQ[qs][q], qxp1 = sQ[qs][q + 1], qxp2 = sQ[qs][q + 2];
-  const float AH = k.AH, BH = k.BH, BC = k.BC, C = k.CH, DT = k.DT;
-  const float4 s0 = k.stab[3 * jm], s1 = k.stab[3 * jm + 1], s2 = k.stab[3 * jm + 2];
-  const float exz = AH * (up1 - um2 + wxp1 - wxm2) + BH * (u0 - um1 + w0 - wxm1);
-  const float exx = AH * (uxp2 - uxm1) + BH * (uxp1 - u0);
-  const float qxx = AH * (qxp2 - qxm1) + BH * (qxp1 - qx0);
-  float ezz = (C * (wp1 - w0));
-  float qzz = (C * (zp1 - z0));
-  ezz = AH * (wp2 - wm1) + BC * ezz;
-  qzz = AH * (zp2 - zm1) + BC * qzz;
-  const float rxx = s0.y * exx + s0.z * ezz + s0.w * qxx + s0.w * qzz;
-  const float rzz = s1.x * exx + s1.y * ezz + s1.z * qxx + s1.z * qzz;
-  const float rpp = s1.w * exx + s2.x * ezz + s2.y * qxx + s2.y * qzz;
-  o[2] = txz + DT * (s0.x * exz);
-  if (i == k.LIN_X && s == k.LIN_Z) {
-    const float2 ds = k.dsrc[itile - 1];
-    o[0] = txx + DT * (rxx - ds.x);
-    o[1] = tzz + DT * (rzz - ds.x);
-    o[3] = pr + DT * (rpp + ds.y);
-  } else {
-    o[0] = txx + DT * rxx;
-    o[1] = tzz + DT * rzz;
-    o[3] = pr + DT * rpp;
-  }
-}
-
-struct FusedLds {
-  float sU[5][kLW], sW[6][kLW], sZ[6][kLW], sQ[4][kLW];
-  float nX[3][kSW], nXZ[3][kSW], nP[3][kSW];
+// LDS of one march of W columns: old-velocity rings (U 5 rows, W and QZ 6,
+// QX 4; columns c-4 .. c+W+3) and new-stress rings (TXX, TXZ, PRES 3 rows;
+// columns c-2 .. c+W+1).
+template <int W>
+struct MarchLds {
+  static constexpr int LW = W + 8, SW = W + 4;
+  static constexpr int kFloats = 21 * LW + 9 * SW;
 };
 
-template <int NR>
-__device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, int b, int itile,
-                                            FusedLds &L) {
-  auto &sU = L.sU;
-  auto &sW = L.sW;
-  auto &sZ = L.sZ;
-  auto &sQ = L.sQ;
-  auto &nX = L.nX;
-  auto &nXZ = L.nXZ;
-  auto &nP = L.nP;
-  const int t = threadIdx.x;
-  const int bx = b % fu.nfx, bz = b / fu.nfx;
-  const int c = fu.vx0 + bx * kFW;
-  const int a = fu.vz0 + bz * NR;
-  const int e = min(a + NR, fu.vz1);
+// A k_step block holds one full tile (kFW columns), two half tiles or four
+// quarter tiles, each march with LDS of its own.
+struct FusedLds {
+  static constexpr int kMax1 = MarchLds<kFW>::kFloats, kMax2 = 2 * MarchLds<kFW / 2>::kFloats;
+  static constexpr int kMax4 = 4 * MarchLds<kFW / 4>::kFloats;
+  static constexpr int kFloats = kMax1 > kMax2 ? (kMax1 > kMax4 ? kMax1 : kMax4) : (kMax2 > kMax4 ? kMax2 : kMax4);
+  float raw[kFloats];
+};
+
+// One march of the fused step over columns [c, c+W) and rows [a, e) by W
+// threads (t = 0 .. W-1), with LDS at lds. NS: rows of a march of full
+// height. UNI: the march runs for NS rows whatever e is, doing no work past
+// e, so that several marches of one block meet the same barriers.
+template <int W, int NS, bool UNI>
+__device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int e, int itile, int t,
+                                            float *lds) {
+  constexpr int LW = MarchLds<W>::LW, SW = MarchLds<W>::SW;
+  float (*sU)[LW] = reinterpret_cast<float (*)[LW]>(lds);
+  float (*sW)[LW] = sU + 5;
+  float (*sZ)[LW] = sW + 6;
+  float (*sQ)[LW] = sZ + 6;
+  float (*nX)[SW] = reinterpret_cast<float (*)[SW]>(lds + 21 * LW);
+  float (*nXZ)[SW] = nX + 3;
+  float (*nP)[SW] = nXZ + 3;
+  const bool live = UNI ? a < e : true;         // an empty march only meets the barriers
+  const int last = UNI ? a + NS + 1 : e + 1;    // the loop's last row
   const int P = k.P, OFF = k.OFF;
   const int i = c + t;
   const int gx = i + OFF;
   const bool hl = t < 8;                         // loads one extra old-velocity column
-  const int hq = t < 4 ? t : kFW + t;            // its LDS column (column c-4+hq)
+  const int hq = t < 4 ? t : W + t;              // its LDS column (column c-4+hq)
   const int hgx = c - 4 + hq + OFF;
   const bool hs = t < 4;                         // forms one extra column of new stresses
-  const int hcol = t < 2 ? c - 2 + t : c + kFW + (t - 2);
+  const int hcol = t < 2 ? c - 2 + t : c + W + (t - 2);
   const int hgs = hcol + OFF;
   const float *__restrict__ U = k.U0;
-  const float *__restrict__ W = k.W0;
+  const float *__restrict__ Wv = k.W0;
   const float *__restrict__ QX = k.QX0;
   const float *__restrict__ QZ = k.QZ0;
   const int *__restrict__ JM = k.JM;
   // prefill the rings: U rows a-4..a-2, W and QZ rows a-3..a-1
-  for (int r = a - 4; r <= a - 2; ++r) {
-    sU[r % 5][t + 4] = U[r * P + gx];
-    sW[(r + 1) % 6][t + 4] = W[(r + 1) * P + gx];
-    sZ[(r + 1) % 6][t + 4] = QZ[(r + 1) * P + gx];
-    if (hl) {
-      sU[r % 5][hq] = U[r * P + hgx];
-      sW[(r + 1) % 6][hq] = W[(r + 1) * P + hgx];
-      sZ[(r + 1) % 6][hq] = QZ[(r + 1) * P + hgx];
+  if (live)
+    for (int r = a - 4; r <= a - 2; ++r) {
+      sU[r % 5][t + 4] = U[r * P + gx];
+      sW[(r + 1) % 6][t + 4] = Wv[(r + 1) * P + gx];
+      sZ[(r + 1) % 6][t + 4] = QZ[(r + 1) * P + gx];
+      if (hl) {
+        sU[r % 5][hq] = U[r * P + hgx];
+        sW[(r + 1) % 6][hq] = Wv[(r + 1) * P + hgx];
+        sZ[(r + 1) % 6][hq] = QZ[(r + 1) * P + hgx];
+      }
     }
-  }
   float wxz0 = 0, wxz1 = 0, wxz2 = 0, wxz3 = 0;             // new TXZ rows s-3..s
   float wzz0 = 0, wzz1 = 0, wzz2 = 0, wzz3 = 0, wzz4 = 0;   // new TZZ rows s-4..s
   float wp0 = 0, wp1 = 0, wp2 = 0, wp3 = 0, wp4 = 0;        // new PRES rows s-4..s
   int jm1 = 0, jm2 = 0, jm0 = 0;                            // JM rows s-2, s-1, s
   // values of row s, loaded one iteration ahead so that the loads of the next
   // row are in flight while this row is computed
-  float pu, pw, pz, pq, fhu = 0, fhw = 0, fhz = 0, fhq = 0;
-  float ftxx, ftzz, ftxz, fpr, fhtxx = 0, fhtzz = 0, fhtxz = 0, fhpr = 0;
-  int fjm, fhjm = 0;
+  float pu = 0, pw = 0, pz = 0, pq = 0, fhu = 0, fhw = 0, fhz = 0, fhq = 0;
+  float ftxx = 0, ftzz = 0, ftxz = 0, fpr = 0, fhtxx = 0, fhtzz = 0, fhtxz = 0, fhpr = 0;
+  int fjm = 0, fhjm = 0;
   auto fetch = [&](int s) {
     const int r = s * P;
-    pu = U[r + P + gx]; pw = W[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
-    if (hl) { fhu = U[r + P + hgx]; fhw = W[r + 2 * P + hgx]; fhz = QZ[r + 2 * P + hgx]; fhq = QX[r + hgx]; }
+    pu = U[r + P + gx]; pw = Wv[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
+    if (hl) { fhu = U[r + P + hgx]; fhw = Wv[r + 2 * P + hgx]; fhz = QZ[r + 2 * P + hgx]; fhq = QX[r + hgx]; }
     ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
     fjm = JM[r + gx] - 1;
     if (hs) {
@@ -850,57 +839,65 @@ __device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, i
       fhpr = k.PR0[r + hgs]; fhjm = JM[r + hgs] - 1;
     }
   };
-  fetch(a - 2);
-  for (int s = a - 2; s <= e + 1; ++s) {
+  if (live) fetch(a - 2);
+  for (int s = a - 2; s <= last; ++s) {
+    const bool act = UNI ? live && s <= e + 1 : true;   // this row has work
     const int r = s * P;
-    // ---- old velocities entering the rings, old stresses and index of row s
-    sU[(s + 1) % 5][t + 4] = pu;
-    sW[(s + 2) % 6][t + 4] = pw;
-    sZ[(s + 2) % 6][t + 4] = pz;
-    sQ[s & 3][t + 4] = pq;
-    if (hl) {
-      sU[(s + 1) % 5][hq] = fhu;
-      sW[(s + 2) % 6][hq] = fhw;
-      sZ[(s + 2) % 6][hq] = fhz;
-      sQ[s & 3][hq] = fhq;
+    float otxx = 0, otzz = 0, otxz = 0, opr = 0, htxx = 0, htzz = 0, htxz = 0, hpr = 0;
+    int hjm = 0;
+    if (act) {
+      // ---- old velocities entering the rings, old stresses and index of row s
+      sU[(s + 1) % 5][t + 4] = pu;
+      sW[(s + 2) % 6][t + 4] = pw;
+      sZ[(s + 2) % 6][t + 4] = pz;
+      sQ[s & 3][t + 4] = pq;
+      if (hl) {
+        sU[(s + 1) % 5][hq] = fhu;
+        sW[(s + 2) % 6][hq] = fhw;
+        sZ[(s + 2) % 6][hq] = fhz;
+        sQ[s & 3][hq] = fhq;
+      }
+      otxx = ftxx; otzz = ftzz; otxz = ftxz; opr = fpr;
+      jm2 = jm1;
+      jm1 = jm0;
+      jm0 = fjm;
+      htxx = fhtxx; htzz = fhtzz; htxz = fhtxz; hpr = fhpr;
+      hjm = fhjm;
+      if (s <= e) fetch(s + 1);
     }
-    const float otxx = ftxx, otzz = ftzz, otxz = ftxz, opr = fpr;
-    jm2 = jm1;
-    jm1 = jm0;
-    jm0 = fjm;
-    const float htxx = fhtxx, htzz = fhtzz, htxz = fhtxz, hpr = fhpr;
-    const int hjm = fhjm;
-    if (s <= e) fetch(s + 1);
     __syncthreads();
     // ---- new stresses of row s
-    float o[4];
-    fused_stress(k, sU, sW, sZ, sQ, s, t + 4, i, otxx, otzz, otxz, opr, jm0, itile, o);
-    const int ns = s % 3;
-    nX[ns][t + 2] = o[0];
-    nXZ[ns][t + 2] = o[2];
-    nP[ns][t + 2] = o[3];
-    if (s >= a && s < e) {
-      k.TXX1[r + gx] = o[0];
-      k.TZZ1[r + gx] = o[1];
-      k.TXZ1[r + gx] = o[2];
-      k.PR1[r + gx] = o[3];
-    }
-    wxz0 = wxz1; wxz1 = wxz2; wxz2 = wxz3; wxz3 = o[2];
-    wzz0 = wzz1; wzz1 = wzz2; wzz2 = wzz3; wzz3 = wzz4; wzz4 = o[1];
-    wp0 = wp1; wp1 = wp2; wp2 = wp3; wp3 = wp4; wp4 = o[3];
-    if (hs) {
-      float ho[4];
-      fused_stress(k, sU, sW, sZ, sQ, s, hcol - (c - 4), hcol, htxx, htzz, htxz, hpr, hjm, itile, ho);
-      const int hn = hcol - (c - 2);
-      nX[ns][hn] = ho[0];
-      nXZ[ns][hn] = ho[2];
-      nP[ns][hn] = ho[3];
+    if (act) {
+      float o[4];
+      ring_stress<5, 6, 6, 4, LW>(k, sU, sW, sZ, sQ, s, t + 4, i, otxx, otzz, otxz, opr, jm0, itile, o);
+      const int ns = s % 3;
+      nX[ns][t + 2] = o[0];
+      nXZ[ns][t + 2] = o[2];
+      nP[ns][t + 2] = o[3];
+      if (s >= a && s < e) {
+        k.TXX1[r + gx] = o[0];
+        k.TZZ1[r + gx] = o[1];
+        k.TXZ1[r + gx] = o[2];
+        k.PR1[r + gx] = o[3];
+      }
+      wxz0 = wxz1; wxz1 = wxz2; wxz2 = wxz3; wxz3 = o[2];
+      wzz0 = wzz1; wzz1 = wzz2; wzz2 = wzz3; wzz3 = wzz4; wzz4 = o[1];
+      wp0 = wp1; wp1 = wp2; wp2 = wp3; wp3 = wp4; wp4 = o[3];
+      if (hs) {
+        float ho[4];
+        ring_stress<5, 6, 6, 4, LW>(k, sU, sW, sZ, sQ, s, hcol - (c - 4), hcol, htxx, htzz, htxz, hpr,
+                                    hjm, itile, ho);
+        const int hn = hcol - (c - 2);
+        nX[ns][hn] = ho[0];
+        nXZ[ns][hn] = ho[2];
+        nP[ns][hn] = ho[3];
+      }
     }
     __syncthreads();
     // ---- new velocities of row l = s-2 (INH_QX_INT, INH_V_UV_INT, INH_QZ_INT,
     //      INH_V_W_INT: the expressions of velocity_run)
     const int l = s - 2;
-    if (l >= a && l < e) {
+    if (act && l >= a && l < e) {
       const int sl = l % 3, rl = l * P;
       const float A = k.A, B = k.B;
       const float xm2 = nX[sl][t], xm1 = nX[sl][t + 1], x0 = nX[sl][t + 2], xp1 = nX[sl][t + 3];
@@ -921,6 +918,38 @@ __device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, i
       k.W1[rl + gx] = w + v2.y * qz + v2.z * sz + v2.w * pz;
     }
   }
+}
+
+// Block b of the fused launch (kFW threads): a full tile (kFW columns, NR
+// rows), or the two half tiles (kFW/2 columns, NR/2 rows) or four quarter
+// tiles (kFW/4 columns, NR/4 rows) of one block of the narrow columns.
+template <int NR>
+__device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, int b, int itile,
+                                            FusedLds &L) {
+  const int nfull = fu.nfx * fu.nfz;
+  if (b < nfull) {
+    const int bx = b % fu.nfx, bz = b / fu.nfx;
+    const int a = fu.vz0 + bz * NR;
+    fused_march<kFW, NR, false>(k, fu.vx0 + bx * kFW, a, min(a + NR, fu.vz1), itile, threadIdx.x,
+                                L.raw);
+    return;
+  }
+  b -= nfull;
+  const int c2 = fu.vx0 + kFW * fu.nfx;
+  if (b < fu.nb2) {
+    constexpr int W = kFW / 2, H = NR / 2;
+    const int j = threadIdx.x / W;
+    const int a = fu.vz0 + (2 * b + j) * H;
+    fused_march<W, H, true>(k, c2, a, min(a + H, fu.vz1), itile, threadIdx.x % W,
+                            L.raw + j * MarchLds<W>::kFloats);
+    return;
+  }
+  b -= fu.nb2;
+  constexpr int W = kFW / 4, H = NR / 4;
+  const int j = threadIdx.x / W;
+  const int a = fu.vz0 + (4 * b + j) * H;
+  fused_march<W, H, true>(k, c2 + (kFW / 2) * fu.h2, a, min(a + H, fu.vz1), itile, threadIdx.x % W,
+                          L.raw + j * MarchLds<W>::kFloats);
 }
 
 // Receiver samples of the fields a step left behind (store_seismograms.F90:58-83,
@@ -957,61 +986,82 @@ __device__ __forceinline__ void edges_block(const KParams &k, const int bx, floa
   const int T = k.T, MX = k.MX, NX = k.NX;
   const int ix0 = bx * kEdgeBlock;
   const int ix = ix0 + threadIdx.x;
+  const int i = ix + k.PMLL;
+  const bool col = ix < NX;
+  const bool si = k.SI != 0;
+  const int lst = k.PMLZH;
+  // Every value this block reads is loaded first: rows 2, 3 (written by the
+  // velocity half-step before this launch), rows PMLZH-2 .. PMLZH and the
+  // history, none of which this launch writes (rows 1 and PMLZH >= 4 are its
+  // only field stores), so the loads are in flight together.
+  float* const F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
+  float bf0[4] = {0, 0, 0, 0}, bf1[4] = {0, 0, 0, 0}, bf2[4] = {0, 0, 0, 0};
+  float bsc[4][3] = {}, bsp[4][3] = {};
+  if (col) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+      const float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+      bf0[f] = FA(F[f], i, lst);
+      bf1[f] = FA(F[f], i, lst - 1);
+      bf2[f] = FA(F[f], i, lst - 2);
+      bsc[f][0] = sc[0]; bsc[f][1] = sc[1]; bsc[f][2] = sc[2];
+      bsp[f][0] = sp[0]; bsp[f][1] = sp[1]; bsp[f][2] = sp[2];
+    }
+  }
+  const bool p2 = si && col && i >= 3 - T && i <= MX - 2 + T;
+  float up2 = 0, u2 = 0, w3 = 0, qz3 = 0, qxp2 = 0, qx2 = 0;
+  float2 aux = make_float2(0.0f, 0.0f);
+  if (p2) {
+    up2 = FA(k.U1, i + 1, 2); u2 = FA(k.U1, i, 2);
+    w3 = FA(k.W1, i, 3); qz3 = FA(k.QZ1, i, 3);
+    qxp2 = FA(k.QX1, i + 1, 2); qx2 = FA(k.QX1, i, 2);
+    aux = k.atab[k.JM[ix] - 1];  // JM(I,0)
+  }
   // phase 1: U(i,1), QX(i,1) for the block's columns and one halo column
   // (stress imaging only; without it row 1 is a per-cell row of k_velocity)
-  const bool si = k.SI != 0;
   for (int t = threadIdx.x; si && t <= kEdgeBlock; t += kEdgeBlock) {
     const int jx = ix0 + t;
-    const int i = jx + k.PMLL;
-    if (jx < NX && i >= 3 - T && i <= MX - 1 + T) {
-      const float u1 = FA(k.U1, i, 2) + FA(k.W1, i, 2) - FA(k.W1, i - 1, 2);
-      const float q1 = FA(k.QX1, i, 2) + FA(k.QZ1, i, 2) - FA(k.QZ1, i - 1, 2);
+    const int c = jx + k.PMLL;
+    if (jx < NX && c >= 3 - T && c <= MX - 1 + T) {
+      const float u1 = FA(k.U1, c, 2) + FA(k.W1, c, 2) - FA(k.W1, c - 1, 2);
+      const float q1 = FA(k.QX1, c, 2) + FA(k.QZ1, c, 2) - FA(k.QZ1, c - 1, 2);
       su[t] = u1;
       sq[t] = q1;
     }
   }
   if (si) __syncthreads();
-  if (ix < NX) {
-    const int i = ix + k.PMLL;
+  if (col) {
     if (si && i >= 3 - T && i <= MX - 1 + T) {
       FA(k.U1, i, 1) = su[threadIdx.x];
       FA(k.QX1, i, 1) = sq[threadIdx.x];
     }
     // phase 2
-    if (si && i >= 3 - T && i <= MX - 2 + T) {
-      const int jm = k.JM[ix] - 1;  // JM(I,0)
-      const float2 aux = k.atab[jm];
+    if (p2) {
       const float u0 = su[threadIdx.x], u1 = su[threadIdx.x + 1];
       const float q0 = sq[threadIdx.x], q1 = sq[threadIdx.x + 1];
-      const float du = u1 - u0 + FA(k.U1, i + 1, 2) - FA(k.U1, i, 2);
-      FA(k.W1, i, 1) = FA(k.W1, i, 3) + aux.x * du;
-      FA(k.QZ1, i, 1) = FA(k.QZ1, i, 3) + aux.y * du + (q1 - q0 + FA(k.QX1, i + 1, 2) - FA(k.QX1, i, 2));
+      const float du = u1 - u0 + up2 - u2;
+      FA(k.W1, i, 1) = w3 + aux.x * du;
+      FA(k.QZ1, i, 1) = qz3 + aux.y * du + (q1 - q0 + qxp2 - qx2);
     }
     // bottom row
-    const int lst = k.PMLZH;
-    if (i >= 1 - T && i <= MX + T) {
-      float *F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
+    const bool upd = i >= 1 - T && i <= MX + T;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        const float *c = (f == 1 || f == 3) ? k.abw : k.abh;
-        float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
-        float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
-        const float f1 = FA(F[f], i, lst - 1), f2 = FA(F[f], i, lst - 2);
-        const float n = c[0] * f1 + c[1] * f2 + c[2] * sc[0] + c[3] * sc[1] + c[4] * sc[2] +
-                        c[5] * sp[0] + c[6] * sp[1] + c[7] * sp[2];
+    for (int f = 0; f < 4; ++f) {
+      const float *c = (f == 1 || f == 3) ? k.abw : k.abh;
+      float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+      float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+      const float f1 = bf1[f], f2 = bf2[f];
+      sp[0] = bsc[f][0]; sp[1] = bsc[f][1]; sp[2] = bsc[f][2];
+      if (upd) {
+        const float n = c[0] * f1 + c[1] * f2 + c[2] * bsc[f][0] + c[3] * bsc[f][1] +
+                        c[4] * bsc[f][2] + c[5] * bsp[f][0] + c[6] * bsp[f][1] + c[7] * bsp[f][2];
         FA(F[f], i, lst) = n;
-        sp[0] = sc[0]; sp[1] = sc[1]; sp[2] = sc[2];
         sc[0] = n; sc[1] = f1; sc[2] = f2;
-      }
-    } else {
-      // columns outside the bottom update keep their values; the history still
-      // records them (time_loop.F90:140-153 copies whole rows)
-      float *F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
-      for (int f = 0; f < 4; ++f) {
-        float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
-        float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
-        sp[0] = sc[0]; sp[1] = sc[1]; sp[2] = sc[2];
-        sc[0] = FA(F[f], i, lst); sc[1] = FA(F[f], i, lst - 1); sc[2] = FA(F[f], i, lst - 2);
+      } else {
+        // columns outside the bottom update keep their values; the history
+        // still records them (time_loop.F90:140-153 copies whole rows)
+        sc[0] = bf0[f]; sc[1] = f1; sc[2] = f2;
       }
     }
   }

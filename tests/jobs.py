@@ -106,3 +106,23 @@ def wide_deep(steps=160, tpml=9, mx=1100, mz=170, f0=30.0, stress_imaging=True) 
 
 GPU_EXTRA["wide"] = wide_deep
 GPU_EXTRA["wide_nosi"] = lambda: wide_deep(stress_imaging=False)
+
+
+def narrow_tiles(steps=140, tpml=10, mx=950, mz=150, f0=35.0, stress_imaging=True) -> Job:
+    """Interior wide enough for one full fused tile (512 columns), a column of
+    half tiles (256) and one of quarter tiles (128), then a masked run of 30
+    columns: with TPML = 10 the tiles cover [23, 535), [535, 791), [791, 919)
+    and the runs [919, 949). The source cell (790, 100) is the last column of
+    the half tiles."""
+    job = models.layered_job(name="NARROW", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=4, seed=21,
+                             tpml=tpml, f0=f0, n_rec=60, field_std=0.08, corr_len=16.0, levels=5)
+    job.stress_imaging = stress_imaging
+    job.x_source = 790.0 * 2.0
+    job.z_source = (100.0 - (2.0 if stress_imaging else 0.0)) * 2.0
+    if not stress_imaging:
+        job.receivers = [(r[0], 0, r[2], 0) for r in job.receivers]
+    return job
+
+
+GPU_EXTRA["narrow"] = narrow_tiles
+GPU_EXTRA["narrow_nosi"] = lambda: narrow_tiles(stress_imaging=False)

@@ -25,27 +25,62 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
   std::vector<int> cs((size_t)NX * NZ, 0), cv((size_t)NX * NZ, 0);
   auto at = [&](std::vector<int> &v, int x, int z) -> int & { return v[(size_t)z * NX + (x - g.PMLL)]; };
   auto inside = [&](int x, int z) { return x >= g.PMLL && x <= g.PMLXH && z >= 0 && z <= g.PMLZH; };
-  // fused tiles
+  // fused tiles: full tiles, then the column of half tiles and the column of
+  // quarter tiles (several to a block, consecutive in z; the last block's may
+  // run past the rows and then be empty)
   const Fused &f = p.fu;
   long long nf = 0;
+  Fused f2 = f;
+  fused_rows(f2, fr);
+  if (f2.nfz != f.nfz || f2.nb2 != f.nb2 || f2.nb4 != f.nb4) return fail("fused rows", MX, MZ, T, 0, 0);
+  if (f.vxe != f.vx0 + fw * f.nfx + (fw / 2) * f.h2 + (fw / 4) * f.h4)
+    return fail("fused columns", MX, MZ, T, f.vxe, 0);
+  auto tile = [&](int c, int wd, int a, int e) -> int {
+    if (c < 5 || c + wd > MX - 3 || a < 5 || e > MZ - 3) return fail("fused tile range", MX, MZ, T, c, a);
+    if ((c + off) % 32 != 0) return fail("fused alignment", MX, MZ, T, c, a);
+    // reads: columns c-4 .. c+wd+3, rows a-4 .. e+3
+    if (!inside(c - 4, a - 4) || !inside(c + wd + 3, e + 3)) return fail("fused reads", MX, MZ, T, c, a);
+    for (int z = a; z < e; ++z)
+      for (int x = c; x < c + wd; ++x) { ++at(cs, x, z); ++at(cv, x, z); ++nf; }
+    return 0;
+  };
   for (int bz = 0; bz < f.nfz; ++bz)
     for (int bx = 0; bx < f.nfx; ++bx) {
       const int c = f.vx0 + bx * fw, a = f.vz0 + bz * fr;
-      const int e = std::min(a + fr, f.vz1), wd = fw;
-      if (e <= a || c + wd > f.vxe) return fail("empty fused tile", MX, MZ, T, c, a);
-      if (c < 5 || c + wd > MX - 3 || a < 5 || e > MZ - 3) return fail("fused tile range", MX, MZ, T, c, a);
-      // reads: columns c-4 .. c+fw+3, rows a-4 .. e+3
-      if (!inside(c - 4, a - 4) || !inside(c + fw + 3, e + 3)) return fail("fused reads", MX, MZ, T, c, a);
-      for (int z = a; z < e; ++z)
-        for (int x = c; x < c + wd; ++x) { ++at(cs, x, z); ++at(cv, x, z); ++nf; }
+      const int e = std::min(a + fr, f.vz1);
+      if (e <= a || c + fw > f.vxe) return fail("empty fused tile", MX, MZ, T, c, a);
+      if (tile(c, fw, a, e)) return 1;
     }
+  for (int part = 0; part < 2; ++part) {
+    const int per = part ? 4 : 2, wd = fw / per, hr = fr / per, nb = part ? f.nb4 : f.nb2;
+    const int c = f.vx0 + fw * f.nfx + (part ? (fw / 2) * f.h2 : 0);
+    if ((part ? f.h4 : f.h2) == 0) {
+      if (nb) return fail("narrow blocks without a column", MX, MZ, T, c, 0);
+      continue;
+    }
+    for (int b = 0; b < nb; ++b)
+      for (int j = 0; j < per; ++j) {
+        const int a = f.vz0 + (per * b + j) * hr, e = std::min(a + hr, f.vz1);
+        if (e <= a) {
+          if (b != nb - 1) return fail("empty narrow tile", MX, MZ, T, c, a);
+          continue;
+        }
+        if (tile(c, wd, a, e)) return 1;
+      }
+  }
   if (nf != p.fused_cells) return fail("fused count", MX, MZ, T, 0, 0);
   auto runs = [&](const Split &s, std::vector<int> &cnt, int zmin, long long *n) -> int {
     for (int q = 0; q < s.nrect; ++q) {
-      const int x0 = s.qx0[q], x1 = x0 + 64 * s.qntx[q];
+      // stores [x0, qxl); the lanes that use their neighbours' values reach
+      // columns x0-2 .. qxl+1, rows qz0-2 .. qz1+2 (loads past the grid's last
+      // column are clamped to it and feed only lanes that do not store)
+      const int x0 = s.qx0[q], x1 = s.qxl[q];
       if ((x0 + off) % 32 != 0) return fail("run alignment", MX, MZ, T, x0, s.qz0[q]);
       if (x0 < 4 || x1 > MX - 1 || s.qz0[q] < zmin || s.qz1[q] > MZ - 1)
         return fail("run range", MX, MZ, T, x0, s.qz0[q]);
+      if (s.qntx[q] != (x1 - x0 + 63) / 64) return fail("run strips", MX, MZ, T, x0, s.qz0[q]);
+      if (!inside(x0 - 2, s.qz0[q] - 2) || !inside(x1 + 1, std::min(s.qz1[q] + 2, g.PMLZH)))
+        return fail("run reads", MX, MZ, T, x0, s.qz0[q]);
       const int nz = (s.qz1[q] - s.qz0[q] + 4 * s.nr - 1) / (4 * s.nr);
       if (s.qblk[q] != s.qntx[q] * nz || s.qcum[q + 1] != s.qcum[q] + s.qblk[q])
         return fail("run blocks", MX, MZ, T, x0, s.qz0[q]);
@@ -110,7 +145,8 @@ static int check2(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv)
     long long &n = pass ? nr_v : nr_s;
     const int zmin = pass ? 2 : 5;
     for (int q = 0; q < s.nrect; ++q) {
-      const int x0 = s.qx0[q], x1 = x0 + 64 * s.qntx[q];
+      const int x0 = s.qx0[q], x1 = s.qxl[q];
+      if (x1 != x0 + 64 * s.qntx[q]) return fail("two-step run strips", MX, MZ, T, x0, s.qz0[q]);
       if ((x0 + off) % 32 != 0) return fail("run alignment", MX, MZ, T, x0, s.qz0[q]);
       if (x0 < 4 || x1 > MX - 1 || s.qz0[q] < zmin || s.qz1[q] > MZ - 1)
         return fail("run range", MX, MZ, T, x0, s.qz0[q]);
