@@ -62,7 +62,8 @@ __global__ __launch_bounds__(kStepBlock) void k_step(KParams k, Fused fu, Split 
   __shared__ FusedLds L;
   int b = blockIdx.x;
   if (b < ln.nfused) {
-    fused_block<FR>(k, fu, xcd_tile(b, ln.nfused), itile, L);
+    const int item = fused_item(fu, b);
+    if (item >= 0) fused_block<FR>(k, fu, item, itile, L);
     return;
   }
   // the other parts work in kBlock threads; the rest of the block has no work
@@ -609,7 +610,7 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   const GridDesc &g = h->g;
   const dim3 blk(kBlock);
   Launch1 ln{};
-  ln.nfused = bq::fused_blocks(h->fu);
+  ln.nfused = bq::fused_blocks(h->fu) ? bq::fused_launch_blocks(h->fu) : 0;
   ln.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
   ln.nruns = h->ss.qcum[kMaxRuns];
   const int vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;

@@ -148,12 +148,15 @@ static long long split_cells(const Split &s) {
   return n;
 }
 
+// Fewest columns a masked quarter column takes; narrower rests go to the runs.
+constexpr int kMinMasked = 32;
+
 void fused_rows(Fused &fu, int fr) {
   const int rows = fu.vz1 - fu.vz0;
   auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
   fu.nfz = (fu.nfx > 0 && rows > 0) ? cdiv(rows, fr) : 0;
   fu.nb2 = (fu.h2 && rows > 0) ? cdiv(cdiv(rows, fr / 2), 2) : 0;
-  fu.nb4 = (fu.h4 && rows > 0) ? cdiv(cdiv(rows, fr / 4), 4) : 0;
+  fu.nb4 = (fu.n4 && rows > 0) ? fu.n4 * cdiv(cdiv(rows, fr / 4), 4) : 0;
 }
 
 // Interior columns start at the first column >= 5 whose address is 128-byte
@@ -162,10 +165,12 @@ void fused_rows(Fused &fu, int fr) {
 // new stresses reach two rows and columns further, still inside [3, MX-2] x
 // [3, MZ'-2], where both half-steps use the interior formulas. Full tiles
 // (fw columns) come first, then a column of half tiles and one of quarter
-// tiles where the rest of [fx, MX-3) holds them. Runs (interior formulas,
+// tiles where the rest of [fx, MX-3) holds them (the last quarter column
+// masked to what is left, when that is 32 columns or more). Runs (interior formulas,
 // 64-column strips, the last one masked) take the columns [vxe, MX-1) right
-// of the tiles and the rows just above and below them; every other cell - the
-// PML strips, the surface and bottom rows, the columns from MX-1 - goes to the
+// of the tiles (none after a masked column, which ends at MX-3) and the rows
+// just above and below the tiles; every other cell - the PML strips, the
+// surface and bottom rows, the columns right of the runs - goes to the
 // per-cell path. The runs read rows up to z1+2 and columns x0-2 .. x0+64*ntx+1,
 // the columns clamped to the grid's last column; with z1 <= MZ'-1, TPML >= 1,
 // 4 <= x0 and qxl <= MX-1 the values they use stay inside the grid.
@@ -175,7 +180,6 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   int fx = 5;
   while ((fx + off) % 32 != 0) ++fx;
   const int ntx = g.MX - 2 >= fx ? (g.MX - 2 - fx + 1) / 64 : 0;
-  const int xr = g.MX - 1;  // runs end (exclusive)
   Fused &fu = p.fu;
   fu.vx0 = fx;
   fu.vz0 = 5;
@@ -186,8 +190,18 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   int rest = width - fw * fu.nfx;
   fu.h2 = rest >= fw / 2;
   rest -= fu.h2 * (fw / 2);
-  fu.h4 = rest >= fw / 4;
-  fu.vxe = fx + fw * fu.nfx + (fw / 2) * fu.h2 + (fw / 4) * fu.h4;
+  fu.n4 = 0;
+  fu.w4 = fw / 4;
+  if (rest >= fw / 4) {
+    fu.n4 = 1;
+    rest -= fw / 4;
+  }
+  if (rest >= kMinMasked) {  // a last quarter column, masked to the rest
+    fu.n4 += 1;
+    fu.w4 = rest;
+    rest = 0;
+  }
+  fu.vxe = fx + fw * fu.nfx + (fw / 2) * fu.h2 + (fw / 4) * (fu.n4 - (fu.n4 > 0)) + (fu.n4 ? fu.w4 : 0);
   fused_rows(fu, fr);
   if (ntx == 0) {  // everything on the per-cell path
     const Rect cs[1] = {{g.PMLL, PX1, zs, ZS1}};
@@ -197,6 +211,9 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   } else {
     const int vxe = fu.vxe;  // = fx without fused tiles
     const bool any = vxe > fx;
+    // runs end (exclusive): MX-1, or the tiles' end where a masked quarter
+    // column took the interior to MX-3 (the two columns left go per cell)
+    const int xr = (vxe + off) % 32 == 0 ? g.MX - 1 : vxe;
     p.fused_cells = any ? (long long)(vxe - fx) * (fu.vz1 - fu.vz0) : 0;
     const int zf = any ? fu.vz1 : 5;  // first row below the fused tiles
     const Rect rs[2] = {{vxe, xr, 5, g.MZ - 1}, {fx, vxe, zf, g.MZ - 1}};

@@ -60,14 +60,16 @@ struct Split {
 //   nfx columns of full tiles (the tile width fw) from vx0, in nfz rows of
 //   blocks of the block height fr (the last one clipped);
 //   then, where h2 = 1, one column of half tiles (fw/2 wide, fr/2 rows), two
-//   to a block (nb2 blocks), and where h4 = 1, one column of quarter tiles
-//   (fw/4 wide, fr/4 rows), four to a block (nb4 blocks). A block's tiles are
-//   consecutive in z; tiles past vz1 are empty.
-// nfx = h2 = h4 = 0: no fused tiles. The block counts depend on the block
+//   to a block (nb2 blocks), and n4 (0..2) columns of quarter tiles (fw/4
+//   wide, fr/4 rows), four to a block (nb4 blocks for all of them); the last
+//   quarter column stores only its first w4 columns (w4 <= fw/4; its other
+//   threads compute the halo values the stored columns need). A block's tiles
+//   are consecutive in z; tiles past vz1 are empty.
+// No fused columns: nfx = h2 = n4 = 0. The block counts depend on the block
 // height (fused_rows).
 struct Fused {
   int vx0, vxe, nfx, vz0, vz1, nfz;
-  int h2, h4, nb2, nb4;
+  int h2, n4, w4, nb2, nb4;
 };
 
 // Sets nfz, nb2, nb4 of fu for block height fr.
@@ -75,6 +77,16 @@ void fused_rows(Fused &fu, int fr);
 
 // Blocks of a fused launch: full, half and quarter tiles.
 inline int fused_blocks(const Fused &fu) { return fu.nfx * fu.nfz + fu.nb2 + fu.nb4; }
+
+// Blocks of a fused launch in XCD order (fused_item): each of the 8 XCDs
+// takes its share of every class, so the launch is 8 times the largest share
+// (the blocks past an XCD's share do nothing).
+inline int fused_launch_blocks(const Fused &fu) {
+  const int n[3] = {fu.nfx * fu.nfz, fu.nb2, fu.nb4};
+  int l = 0;
+  for (int c = 0; c < 3; ++c) l += (n[c] + 7) / 8;
+  return 8 * l;
+}
 
 struct Plan {
   Fused fu;

@@ -777,12 +777,15 @@ struct FusedLds {
 };
 
 // One march of the fused step over columns [c, c+W) and rows [a, e) by W
-// threads (t = 0 .. W-1), with LDS at lds. NS: rows of a march of full
-// height. UNI: the march runs for NS rows whatever e is, doing no work past
-// e, so that several marches of one block meet the same barriers.
+// threads (t = 0 .. W-1), with LDS at lds; only the first ws columns are
+// stored (ws < W: a masked march, whose other threads form the halo values
+// of the stored columns; their loads are clamped to the grid's last column).
+// NS: rows of a march of full height. UNI: the march runs for NS rows
+// whatever e is, doing no work past e, so that several marches of one block
+// meet the same barriers.
 template <int W, int NS, bool UNI>
-__device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int e, int itile, int t,
-                                            float *lds) {
+__device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int a, int e, int itile,
+                                            int t, float *lds) {
   constexpr int LW = MarchLds<W>::LW, SW = MarchLds<W>::SW;
   float (*sU)[LW] = reinterpret_cast<float (*)[LW]>(lds);
   float (*sW)[LW] = sU + 5;
@@ -793,15 +796,16 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int 
   float (*nP)[SW] = nXZ + 3;
   const bool live = UNI ? a < e : true;         // an empty march only meets the barriers
   const int last = UNI ? a + NS + 1 : e + 1;    // the loop's last row
-  const int P = k.P, OFF = k.OFF;
+  const int P = k.P, OFF = k.OFF, GX = k.NX - 1;
   const int i = c + t;
-  const int gx = i + OFF;
+  const bool own = t < ws;                       // stores its column
+  const int gx = min(i + OFF, GX);
   const bool hl = t < 8;                         // loads one extra old-velocity column
   const int hq = t < 4 ? t : W + t;              // its LDS column (column c-4+hq)
-  const int hgx = c - 4 + hq + OFF;
+  const int hgx = min(c - 4 + hq + OFF, GX);
   const bool hs = t < 4;                         // forms one extra column of new stresses
   const int hcol = t < 2 ? c - 2 + t : c + W + (t - 2);
-  const int hgs = hcol + OFF;
+  const int hgs = min(hcol + OFF, GX);
   const float *__restrict__ U = k.U0;
   const float *__restrict__ Wv = k.W0;
   const float *__restrict__ QX = k.QX0;
@@ -874,7 +878,7 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int 
       nX[ns][t + 2] = o[0];
       nXZ[ns][t + 2] = o[2];
       nP[ns][t + 2] = o[3];
-      if (s >= a && s < e) {
+      if (own && s >= a && s < e) {
         k.TXX1[r + gx] = o[0];
         k.TZZ1[r + gx] = o[1];
         k.TXZ1[r + gx] = o[2];
@@ -897,7 +901,7 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int 
     // ---- new velocities of row l = s-2 (INH_QX_INT, INH_V_UV_INT, INH_QZ_INT,
     //      INH_V_W_INT: the expressions of velocity_run)
     const int l = s - 2;
-    if (act && l >= a && l < e) {
+    if (act && own && l >= a && l < e) {
       const int sl = l % 3, rl = l * P;
       const float A = k.A, B = k.B;
       const float xm2 = nX[sl][t], xm1 = nX[sl][t + 1], x0 = nX[sl][t + 2], xp1 = nX[sl][t + 3];
@@ -920,6 +924,26 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int a, int 
   }
 }
 
+// Fused block of launch block b (block b runs on XCD b % 8): XCD x takes a
+// contiguous band of the full tiles, then of the half-tile blocks, then of
+// the quarter-tile blocks, in that order, so the short blocks are dispatched
+// on every XCD after its long ones and take the slots the half blocks free.
+// -1: no block (past the XCD's share).
+__device__ __forceinline__ int fused_item(const Fused &fu, int b) {
+  const int x = b & 7;
+  int q = b >> 3, base = 0;
+  const int n[3] = {fu.nfx * fu.nfz, fu.nb2, fu.nb4};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int per = n[c] >> 3, rem = n[c] & 7;
+    const int cnt = per + (x < rem);
+    if (q < cnt) return base + x * per + min(x, rem) + q;
+    q -= cnt;
+    base += n[c];
+  }
+  return -1;
+}
+
 // Block b of the fused launch (kFW threads): a full tile (kFW columns, NR
 // rows), or the two half tiles (kFW/2 columns, NR/2 rows) or four quarter
 // tiles (kFW/4 columns, NR/4 rows) of one block of the narrow columns.
@@ -930,8 +954,8 @@ __device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, i
   if (b < nfull) {
     const int bx = b % fu.nfx, bz = b / fu.nfx;
     const int a = fu.vz0 + bz * NR;
-    fused_march<kFW, NR, false>(k, fu.vx0 + bx * kFW, a, min(a + NR, fu.vz1), itile, threadIdx.x,
-                                L.raw);
+    fused_march<kFW, NR, false>(k, fu.vx0 + bx * kFW, kFW, a, min(a + NR, fu.vz1), itile,
+                                threadIdx.x, L.raw);
     return;
   }
   b -= nfull;
@@ -940,15 +964,19 @@ __device__ __forceinline__ void fused_block(const KParams &k, const Fused &fu, i
     constexpr int W = kFW / 2, H = NR / 2;
     const int j = threadIdx.x / W;
     const int a = fu.vz0 + (2 * b + j) * H;
-    fused_march<W, H, true>(k, c2, a, min(a + H, fu.vz1), itile, threadIdx.x % W,
+    fused_march<W, H, true>(k, c2, W, a, min(a + H, fu.vz1), itile, threadIdx.x % W,
                             L.raw + j * MarchLds<W>::kFloats);
     return;
   }
   b -= fu.nb2;
   constexpr int W = kFW / 4, H = NR / 4;
+  const int per = fu.nb4 / fu.n4;    // blocks of one quarter column
+  const int col = b / per;
+  b -= col * per;
   const int j = threadIdx.x / W;
   const int a = fu.vz0 + (4 * b + j) * H;
-  fused_march<W, H, true>(k, c2 + (kFW / 2) * fu.h2, a, min(a + H, fu.vz1), itile, threadIdx.x % W,
+  fused_march<W, H, true>(k, c2 + (kFW / 2) * fu.h2 + W * col, col == fu.n4 - 1 ? fu.w4 : W, a,
+                          min(a + H, fu.vz1), itile, threadIdx.x % W,
                           L.raw + j * MarchLds<W>::kFloats);
 }
 

@@ -126,3 +126,6 @@ def narrow_tiles(steps=140, tpml=10, mx=950, mz=150, f0=35.0, stress_imaging=Tru
 
 GPU_EXTRA["narrow"] = narrow_tiles
 GPU_EXTRA["narrow_nosi"] = lambda: narrow_tiles(stress_imaging=False)
+# 1000 columns: the quarter tiles [791, 919) are followed by a second quarter
+# column masked to its first 78 columns, [919, 997) up to MX-3
+GPU_EXTRA["narrow_masked"] = lambda: narrow_tiles(mx=1000)

@@ -33,15 +33,22 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
   Fused f2 = f;
   fused_rows(f2, fr);
   if (f2.nfz != f.nfz || f2.nb2 != f.nb2 || f2.nb4 != f.nb4) return fail("fused rows", MX, MZ, T, 0, 0);
-  if (f.vxe != f.vx0 + fw * f.nfx + (fw / 2) * f.h2 + (fw / 4) * f.h4)
+  if (f.n4 < 0 || f.n4 > 2 || f.w4 < 1 || f.w4 > fw / 4 || (f.n4 == 0 && f.w4 != fw / 4))
+    return fail("quarter columns", MX, MZ, T, f.vxe, 0);
+  if (f.vxe != f.vx0 + fw * f.nfx + (fw / 2) * f.h2 + (fw / 4) * (f.n4 - (f.n4 > 0)) + (f.n4 ? f.w4 : 0))
     return fail("fused columns", MX, MZ, T, f.vxe, 0);
-  auto tile = [&](int c, int wd, int a, int e) -> int {
-    if (c < 5 || c + wd > MX - 3 || a < 5 || e > MZ - 3) return fail("fused tile range", MX, MZ, T, c, a);
+  // a tile of wd columns stores [c, c+ws); its threads form stresses at
+  // c-2 .. c+wd+1 and read old velocities at c-4 .. c+wd+3, clamped to the
+  // grid's last column: the values the stored cells use (stresses at
+  // c-2 .. c+ws+1, velocities at c-4 .. c+ws+3) must be unclamped, inside the
+  // grid, and of the interior formulas (stresses at columns <= MX-2)
+  auto tile = [&](int c, int wd, int ws, int a, int e) -> int {
+    if (c < 5 || c + ws > MX - 3 || a < 5 || e > MZ - 3) return fail("fused tile range", MX, MZ, T, c, a);
     if ((c + off) % 32 != 0) return fail("fused alignment", MX, MZ, T, c, a);
-    // reads: columns c-4 .. c+wd+3, rows a-4 .. e+3
-    if (!inside(c - 4, a - 4) || !inside(c + wd + 3, e + 3)) return fail("fused reads", MX, MZ, T, c, a);
+    if (!inside(c - 4, a - 4) || !inside(c + ws + 3, e + 3)) return fail("fused reads", MX, MZ, T, c, a);
+    if (ws < wd && c + ws + 1 > MX - 2) return fail("masked tile halo", MX, MZ, T, c, a);
     for (int z = a; z < e; ++z)
-      for (int x = c; x < c + wd; ++x) { ++at(cs, x, z); ++at(cv, x, z); ++nf; }
+      for (int x = c; x < c + ws; ++x) { ++at(cs, x, z); ++at(cv, x, z); ++nf; }
     return 0;
   };
   for (int bz = 0; bz < f.nfz; ++bz)
@@ -49,15 +56,21 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
       const int c = f.vx0 + bx * fw, a = f.vz0 + bz * fr;
       const int e = std::min(a + fr, f.vz1);
       if (e <= a || c + fw > f.vxe) return fail("empty fused tile", MX, MZ, T, c, a);
-      if (tile(c, fw, a, e)) return 1;
+      if (tile(c, fw, fw, a, e)) return 1;
     }
-  for (int part = 0; part < 2; ++part) {
-    const int per = part ? 4 : 2, wd = fw / per, hr = fr / per, nb = part ? f.nb4 : f.nb2;
-    const int c = f.vx0 + fw * f.nfx + (part ? (fw / 2) * f.h2 : 0);
-    if ((part ? f.h4 : f.h2) == 0) {
-      if (nb) return fail("narrow blocks without a column", MX, MZ, T, c, 0);
-      continue;
-    }
+  // half column, then the quarter columns
+  const int rows = f.vz1 - f.vz0;
+  auto cdiv = [](int x, int y) { return (x + y - 1) / y; };
+  if (f.nb2 != (f.h2 ? cdiv(cdiv(rows, fr / 2), 2) : 0) ||
+      f.nb4 != (f.n4 ? f.n4 * cdiv(cdiv(rows, fr / 4), 4) : 0))
+    return fail("narrow blocks", MX, MZ, T, f.vxe, 0);
+  const int ncol = f.h2 + f.n4;
+  for (int q = 0; q < ncol; ++q) {
+    const bool half = q < f.h2;
+    const int per = half ? 2 : 4, wd = fw / per, hr = fr / per;
+    const int nb = half ? f.nb2 : f.nb4 / f.n4;
+    const int c = f.vx0 + fw * f.nfx + (half ? 0 : (fw / 2) * f.h2 + wd * (q - f.h2));
+    const int ws = (!half && q == ncol - 1) ? f.w4 : wd;
     for (int b = 0; b < nb; ++b)
       for (int j = 0; j < per; ++j) {
         const int a = f.vz0 + (per * b + j) * hr, e = std::min(a + hr, f.vz1);
@@ -65,7 +78,7 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
           if (b != nb - 1) return fail("empty narrow tile", MX, MZ, T, c, a);
           continue;
         }
-        if (tile(c, wd, a, e)) return 1;
+        if (tile(c, wd, ws, a, e)) return 1;
       }
   }
   if (nf != p.fused_cells) return fail("fused count", MX, MZ, T, 0, 0);
