@@ -210,7 +210,8 @@ class Simulation:
 
     def kernel_times(self) -> dict:
         """ms per step of each launch: k_step (fused interior + boundary stress +
-        previous step's samples), k_velocity (boundary velocity), k_edges."""
+        previous step's samples), k_velocity (boundary velocity, with the edge
+        rows where its edge blocks take them), k_edges (0 then)."""
         out = np.zeros(3, dtype=np.float64)
         self._chk(self.lib.bq_kernel_times(self.h, _ptr(out)))
         return dict(zip(("k_step", "k_velocity", "k_edges"), map(float, out)))
@@ -222,8 +223,8 @@ class Simulation:
         return dict(zip(("k_fused", "k_stress", "k_velocity"), map(int, out)))
 
     def set_sweep(self, steps: int):
-        """Steps per sweep bq_run may use: 2 (default, where the grid allows)
-        or 1."""
+        """Steps per sweep bq_run may use: 1 (the default) or 2 (where the
+        grid allows)."""
         self._chk(self.lib.bq_set_sweep(self.h, int(steps)))
 
     def sweep(self) -> int:

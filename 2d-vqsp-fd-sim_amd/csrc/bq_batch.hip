@@ -127,21 +127,11 @@ __device__ __forceinline__ void other_item(const BatchArgs &B, int w, int kofs) 
   if (kofs > 0) sample_receivers(k, b * kBlock + threadIdx.x, itile - 1, true);
 }
 
-__device__ __forceinline__ void velocity_item(const BatchArgs &B, int w, int kofs) {
+__device__ __forceinline__ void velocity_item(const BatchArgs &B, int w, int kofs, float *lds) {
   const int s = find_shot(B.cum + C_VEL * (B.nshots + 1), B.nshots, w);
   const ShotDev &sd = B.shots[s];
   const KParams k = sd.kp[(sd.base_par + kofs) & 1];
-  const int b = w - B.cum[C_VEL * (B.nshots + 1) + s];
-  const Split &sp = sd.sv;
-  if (b < sd.vrest) {
-    int i, l;
-    if (rest_cell(sp, b * kBlock + threadIdx.x, &i, &l)) velocity_cell(k, i, l);
-    return;
-  }
-  int tile;
-  const int r = run_rect(sp, xcd_tile(b - sd.vrest, sd.vruns), &tile);
-  velocity_run<kRun>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
-                     threadIdx.x & 63);
+  velocity_block(k, sd.sv, w - B.cum[C_VEL * (B.nshots + 1) + s], sd.vrest, sd.vruns, lds);
 }
 
 // Next item of a persistent block on XCD xcd (block b runs on XCD b % 8):
@@ -205,10 +195,11 @@ __global__ __launch_bounds__(kBlock) void kb_rest(BatchArgs B, int kl) {
 template <bool PERSIST>
 __global__ __launch_bounds__(kBlock, 5) void kb_velocity(BatchArgs B, int kl) {
   __shared__ int item;
+  __shared__ float lds[kEdgeLds];
   const int kofs = *B.dstep + kl;
   const int total = B.total[C_VEL];
   if (!PERSIST) {
-    velocity_item(B, blockIdx.x, kofs);
+    velocity_item(B, blockIdx.x, kofs, lds);
     return;
   }
   for (;;) {
@@ -220,7 +211,7 @@ __global__ __launch_bounds__(kBlock, 5) void kb_velocity(BatchArgs B, int kl) {
     const int w = item;
     __syncthreads();
     if (w < 0) break;
-    velocity_item(B, w, kofs);
+    velocity_item(B, w, kofs, lds);
   }
   if (threadIdx.x == 0) persistent_done(B.queue);
 }
@@ -370,11 +361,11 @@ int layout(bq_batch *b, int fr) {
     d.nsample = (h->mr + kBlock - 1) / kBlock;
     d.vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;
     d.vruns = h->sv.qcum[kMaxRuns];
-    d.nedge = (h->g.NX + kEdgeBlock - 1) / kEdgeBlock;
+    d.nedge = h->sv.fold ? 0 : (h->g.NX + kEdgeBlock - 1) / kEdgeBlock;
     d.lb_on = h->lb != nullptr;
     d.nlb = d.lb_on ? 2 * lb_tiles(lb_lz(h->g.MZ)) + lb_tiles(lb_lx(h->g.MX)) : 0;
     d.nlbx = d.lb_on ? (int)((lb_band_size(h->g.MX, h->g.MZ) + kBlock - 1) / kBlock) : 0;
-    const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample, d.vrest + d.vruns, d.nedge,
+    const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample, h->sv.etop + h->sv.ebot + d.vrest + d.vruns, d.nedge,
                           d.nsample, d.nlb, d.nlbx};
     for (int c = 0; c < C_N; ++c) cum[(size_t)c * (n + 1) + s + 1] = cum[(size_t)c * (n + 1) + s] + cnt[c];
   }
@@ -425,7 +416,8 @@ void launch_step_fr(bq_batch *b, const BatchArgs &a, int kl, int advance) {
       hipLaunchKernelGGL(kb_rest, dim3(a.total[C_OTHER]), dim3(kBlock), 0, b->stream, a, kl);
     if (nvel) hipLaunchKernelGGL((kb_velocity<false>), dim3(nvel), dim3(kBlock), 0, b->stream, a, kl);
   }
-  hipLaunchKernelGGL(kb_edges, dim3(a.total[C_EDGE]), dim3(kEdgeBlock), 0, b->stream, a, kl);
+  if (a.total[C_EDGE])
+    hipLaunchKernelGGL(kb_edges, dim3(a.total[C_EDGE]), dim3(kEdgeBlock), 0, b->stream, a, kl);
   if (a.total[C_LB]) {
     hipLaunchKernelGGL(kb_lb_apply, dim3(a.total[C_LB]), dim3(kLbBlock), 0, b->stream, a, kl);
     hipLaunchKernelGGL(kb_lb_extract, dim3(a.total[C_LBX]), dim3(kBlock), 0, b->stream, a, kl);

@@ -1,19 +1,24 @@
 // bq_device.hip - CDNA4 (gfx950) kernels and C ABI of the Biot time loop.
 //
-// One time step of the reference (time_loop.F90:91-190, serial build, stress
-// imaging) becomes four launches on the handle's stream:
+// One time step of the reference (time_loop.F90:91-190, serial build) is two
+// launches on the handle's stream:
 //
-//   k_stress    rows 2..PMLZH   all strain and stress row kernels of one row
-//                               class fused per cell, plus the image rows 0/1
-//                               of the free surface (HPLANE_0/1_STRESS)
-//   k_velocity  rows 2..PMLZH-1 all INH_* updates of one row class fused per
-//                               cell; the previous filtration velocity is the
-//                               cell's own value before the update, so the
-//                               QXP/QZP copies of the reference are not stored
-//   k_edges     row 1 (HPLANE_1_VELOCITY) and row PMLZH
-//                               (HPLANE_PMLZH0_VELOCITY), which need the new
-//                               neighbouring rows, and the bottom history
-//   k_sample    receiver samples of the step
+//   k_step      the fused interior tiles (full, half and quarter width: both
+//               half-steps of every interior-formula cell in one march), the
+//               stress half-step of every other cell (HPLANE_*_STRESS: PML
+//               strips, surface and bottom rows) and the receiver samples of
+//               the previous step
+//   k_velocity  the velocity half-step of every other cell (HPLANE_*_VELOCITY,
+//               INH_*; the previous filtration velocity is the cell's own
+//               value before the update, so the QXP/QZP copies of the
+//               reference are not stored), with row 1 (HPLANE_1_VELOCITY) and
+//               row PMLZH (HPLANE_PMLZH0_VELOCITY) and the bottom history in
+//               its edge blocks
+//
+// k_edges takes rows 1 and PMLZH in a launch of their own where the velocity
+// launch cannot (two steps per sweep; a PML too wide for the edge blocks);
+// k_sample takes the samples of a run's last step; k_sweep2 and
+// k_rest_stress are the two-step sweep.
 //
 // Each cell is evaluated with the reference's operation order and no
 // floating-point contraction (the file is built with -ffp-contract=off), so
@@ -33,19 +38,14 @@
 
 namespace {
 
-// The velocity half-step of the cells outside the fused blocks: the first
-// nrest blocks take cells one per thread (per-cell path), the others the runs.
+// The velocity half-step of the cells outside the fused blocks: the edge
+// blocks (rows 1-3 and PMLZH-2..PMLZH, where the split folds the edge rows
+// in), then nrest blocks of cells one per thread (per-cell path), then nruns
+// blocks of runs.
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nrest) {
-  if ((int)blockIdx.x < nrest) {
-    int i, l;
-    if (rest_cell(sp, blockIdx.x * kBlock + threadIdx.x, &i, &l)) velocity_cell(k, i, l);
-    return;
-  }
-  int tile;
-  const int r = run_rect(sp, xcd_tile(blockIdx.x - nrest, gridDim.x - nrest), &tile);
-  velocity_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
-                   threadIdx.x & 63);
+__global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nrest, int nruns) {
+  __shared__ float lds[kEdgeLds];
+  velocity_block(k, sp, blockIdx.x, nrest, nruns, lds);
 }
 
 // First launch of a step: fused interior blocks (long-running, first), then
@@ -614,7 +614,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   ln.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
   ln.nruns = h->ss.qcum[kMaxRuns];
   const int vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;
-  const int vgrid = vrest + h->sv.qcum[kMaxRuns];
+  const int vruns = h->sv.qcum[kMaxRuns];
+  const int vgrid = h->sv.etop + h->sv.ebot + vrest + vruns;
   const int nsb = (h->mr + kBlock - 1) / kBlock;
   const dim3 ge((g.NX + kEdgeBlock - 1) / kEdgeBlock);
   // two steps per sweep (bq_get_sweep says when): the partition of the other cells
@@ -623,7 +624,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
   l2.nrest = (h->ss2.cum[kMaxCells] + kBlock - 1) / kBlock;
   l2.nruns = h->ss2.qcum[kMaxRuns];
   const int vrest2 = (h->sv2.cum[kMaxCells] + kBlock - 1) / kBlock;
-  const int vgrid2 = vrest2 + h->sv2.qcum[kMaxRuns];
+  const int vruns2 = h->sv2.qcum[kMaxRuns];
+  const int vgrid2 = h->sv2.etop + h->sv2.ebot + vrest2 + vruns2;
   const int nregion = h->f2.nfx * h->f2.nfz;
   F8 rim{};
   for (int f = 0; f < 8; ++f)
@@ -675,10 +677,12 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       }
       if (e) (void)hipEventRecord(e[1], h->stream);
       if (vgrid)
-        hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest);
+        hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest, vruns);
       if (e) (void)hipEventRecord(e[2], h->stream);
-      hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
-      if (e) (void)hipEventRecord(e[3], h->stream);
+      if (!h->sv.fold) {
+        hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);
+        if (e) (void)hipEventRecord(e[3], h->stream);
+      }
       if (h->lb && bq::lb_after_step(h, k, it)) return fail(h, -100, "learned boundary launch failed");
       last = k;
       it += 1;
@@ -699,7 +703,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     if (gs) hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, kat, h->ss2, l2, it);
     if (e) (void)hipEventRecord(e[2], h->stream);
     if (vgrid2)
-      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, kat, h->sv2, vrest2);
+      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, kat, h->sv2, vrest2,
+                         vruns2);
     if (e) (void)hipEventRecord(e[3], h->stream);
     hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, kat);
     if (e) (void)hipEventRecord(e[4], h->stream);
@@ -711,7 +716,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, ktc, h->ss2, l2, it + 1);
     if (e) (void)hipEventRecord(e[5], h->stream);
     if (vgrid2)
-      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, ktc, h->sv2, vrest2);
+      hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid2), blk, 0, h->stream, ktc, h->sv2, vrest2,
+                         vruns2);
     if (e) (void)hipEventRecord(e[6], h->stream);
     hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, ktc);
     if (e) (void)hipEventRecord(e[7], h->stream);
@@ -734,7 +740,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
     int n1 = 0, n2 = 0;
     for (size_t q = 0; q < timed_kind.size(); ++q) {
       if (timed_kind[q] == 1) {
-        for (int c = 0; c < 3; ++c) s1[c] += dt(q, c, c + 1);
+        // (with the edge rows folded into k_velocity there is no k_edges)
+        for (int c = 0; c < 3; ++c) s1[c] += (c == 2 && h->sv.fold) ? 0.0 : dt(q, c, c + 1);
         ++n1;
       } else {
         s2[0] += dt(q, 0, 1);                          // the two-step sweep
@@ -759,7 +766,8 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
 // Durations (ms per launch) and cells of the last timed run. steps_per_sweep
 // = 2: ms = {two-step sweep, stress of the other cells, velocity, edges},
 // cells = {region, other stress cells, other velocity cells, 0};
-// 1: ms = {k_step, k_velocity, k_edges, 0}, cells = {fused, stress, velocity, 0}.
+// 1: ms = {k_step, k_velocity, k_edges (0 where k_velocity takes the edge
+// rows), 0}, cells = {fused, stress, velocity, 0}.
 int bq_kernel_stats(bq_handle *h, double ms[4], long long cells[4], int *steps_per_sweep) {
   if (!h || !ms || !cells || !steps_per_sweep) return -1;
   if (!h->timed_steps) return fail(h, -6, "no timed run");

@@ -7,6 +7,7 @@
 // order, so the device sees exactly the coefficients the Fortran host would
 // have produced. Built with -ffp-contract=off: every operation rounds as
 // written.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -174,7 +175,26 @@ void fused_rows(Fused &fu, int fr) {
 // per-cell path. The runs read rows up to z1+2 and columns x0-2 .. x0+64*ntx+1,
 // the columns clamped to the grid's last column; with z1 <= MZ'-1, TPML >= 1,
 // 4 <= x0 and qxl <= MX-1 the values they use stay inside the grid.
-Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv) {
+// Edge blocks of the velocity half-step (Split::fold): the last top block
+// starts at elast in [max(PX1 - kEdgeCols, 5), MX-2] and the first ends at
+// PMLL + kEdgeCols >= 5, so the columns next to every boundary between top
+// blocks lie in [4, MX-2], where the velocity formulas are the interior ones
+// and no PML memory is touched; the rows must leave 2, 3 and PMLZH-2,
+// PMLZH-1 apart from each other. Returns false where the grid does not allow it.
+static bool edge_blocks(const GridDesc &g, int zv, Split &sv) {
+  const int PX1 = g.PMLXH + 1;
+  const int last = std::max(PX1 - kEdgeCols, 5);
+  const bool top = zv == 2;  // stress imaging: row 1 from rows 2, 3
+  if (top && (last > g.MX - 2 || g.PMLL + kEdgeCols < 5)) return false;
+  if (g.PMLZH - 2 < (top ? 4 : 2)) return false;
+  sv.fold = 1;
+  sv.etop = top ? 1 + (last - g.PMLL + kEdgeCols - 1) / kEdgeCols : 0;
+  sv.elast = top ? last : 0;
+  sv.ebot = (g.NX + kEdgeCols - 1) / kEdgeCols;
+  return true;
+}
+
+Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold) {
   Plan p{};
   const int PX1 = g.PMLXH + 1, ZS1 = g.PMLZH + 1, ZV1 = g.PMLZH;  // exclusive ends
   int fx = 5;
@@ -203,9 +223,16 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   }
   fu.vxe = fx + fw * fu.nfx + (fw / 2) * fu.h2 + (fw / 4) * (fu.n4 - (fu.n4 > 0)) + (fu.n4 ? fu.w4 : 0);
   fused_rows(fu, fr);
+  // the velocity half-step with its edge rows: rows [zt, zb) are left to the
+  // per-cell and run rectangles, 2 and 3 (stress imaging) to the top blocks,
+  // PMLZH-2 and PMLZH-1 to the bottom blocks
+  Split ev{};
+  const bool fe = fold && edge_blocks(g, zv, ev);
+  const int zt = fe && zv == 2 ? 4 : zv, zb = fe ? g.PMLZH - 2 : ZV1;
+  const int zr = std::max(zt, 2);  // first row of the runs
   if (ntx == 0) {  // everything on the per-cell path
     const Rect cs[1] = {{g.PMLL, PX1, zs, ZS1}};
-    const Rect cv[1] = {{g.PMLL, PX1, zv, ZV1}};
+    const Rect cv[1] = {{g.PMLL, PX1, zt, zb}};
     p.ss = make_split(nullptr, 0, cs, 1, nr);
     p.sv = make_split(nullptr, 0, cv, 1, nr);
   } else {
@@ -220,13 +247,26 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
     const Rect cs[4] = {{g.PMLL, fx, zs, ZS1}, {xr, PX1, zs, ZS1}, {fx, xr, zs, 5},
                         {fx, xr, g.MZ - 1, ZS1}};
     p.ss = make_split(rs, 2, cs, 4, nr);
-    const Rect rv[3] = {{vxe, xr, 2, g.MZ - 1}, {fx, vxe, 2, 5}, {fx, vxe, zf, g.MZ - 1}};
-    const Rect cv[4] = {{g.PMLL, fx, zv, ZV1}, {xr, PX1, zv, ZV1}, {fx, xr, zv, 2},
-                        {fx, xr, g.MZ - 1, ZV1}};
-    p.sv = make_split(rv, 3, cv, 4, nr);
+    const Rect rv[3] = {{vxe, xr, zr, g.MZ - 1}, {fx, vxe, zr, 5}, {fx, vxe, zf, g.MZ - 1}};
+    if (fe) {
+      const Rect cv[4] = {{g.PMLL, fx, zt, g.MZ - 1}, {xr, PX1, zt, g.MZ - 1}, {fx, xr, zt, 2},
+                          {g.PMLL, PX1, g.MZ - 1, zb}};
+      p.sv = make_split(rv, 3, cv, 4, nr);
+    } else {
+      const Rect cv[4] = {{g.PMLL, fx, zv, ZV1}, {xr, PX1, zv, ZV1}, {fx, xr, zv, 2},
+                          {fx, xr, g.MZ - 1, ZV1}};
+      p.sv = make_split(rv, 3, cv, 4, nr);
+    }
   }
   p.stress_cells = split_cells(p.ss);
   p.velocity_cells = split_cells(p.sv);
+  if (fe) {
+    p.sv.fold = 1;
+    p.sv.etop = ev.etop;
+    p.sv.ebot = ev.ebot;
+    p.sv.elast = ev.elast;
+    p.velocity_cells += (long long)g.NX * (ev.etop ? 4 : 2);  // rows 2, 3 and PMLZH-2, PMLZH-1
+  }
   return p;
 }
 

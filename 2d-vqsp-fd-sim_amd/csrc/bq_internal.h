@@ -48,8 +48,18 @@ constexpr int kMaxCells = 8;  // per-cell rectangles of one half-step
 // nr rows (qblk blocks, qcum cumulative). Per-cell path: up to kMaxCells
 // rectangles [rx0, rx1) x [rz0, rz1), one thread per cell, cum the cumulative
 // cell counts.
+// Velocity half-step only (fold = 1): the edge rows come with it. etop top
+// blocks take rows 2 and 3 and then row 1 (stress imaging), each over up to
+// kEdgeCols columns: block k from column PMLL + kEdgeCols*k, the last one from
+// elast to the grid's end, so that every column a block recomputes for its
+// neighbour (one each side, row 2) uses the interior formulas. ebot bottom
+// blocks take rows PMLZH-2 and PMLZH-1 and then row PMLZH with the bottom
+// history, over kEdgeCols columns each.
+constexpr int kEdgeCols = 128;
+
 struct Split {
   int nr, nrect;
+  int fold, etop, ebot, elast;
   int qx0[kMaxRuns], qxl[kMaxRuns], qntx[kMaxRuns], qz0[kMaxRuns], qz1[kMaxRuns], qblk[kMaxRuns];
   int qcum[kMaxRuns + 1];
   int rx0[kMaxCells], rx1[kMaxCells], rz0[kMaxCells], rz1[kMaxCells];
@@ -97,8 +107,10 @@ struct Plan {
 // Partition of the cells of one step: the stress half-step covers x in
 // [PMLL, PMLXH], rows [zs, PMLZH]; the velocity half-step rows [zv, PMLZH-1]
 // (the rows outside these are the kernels' own edge rows). off = -PMLL; fw,
-// fr: fused tile width and height; nr: rows per wavefront run.
-Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv);
+// fr: fused tile width and height; nr: rows per wavefront run. fold: the
+// velocity half-step takes the edge rows too (Split::fold), where the grid
+// allows it.
+Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold = true);
 
 // Two-step region: own cells [x0, x1) x [z0, z1) in nfx x nfz tiles of the
 // tile width and fr rows (the last row of tiles clipped); nfx = 0: none.

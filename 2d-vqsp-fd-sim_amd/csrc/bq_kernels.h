@@ -17,6 +17,7 @@ using bq::GridDesc;
 using bq::Split;
 using bq::kMaxCells;
 using bq::kMaxRuns;
+using bq::kEdgeCols;
 
 // Named (not anonymous) so that the host structures holding KParams have the
 // same type in every source file that includes this header.
@@ -256,7 +257,12 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
 // INH_QX_INT_0, INH_V_UV_INT_0, INH_QZ_INT_0, INH_V_W_INT_0, INH_*_2ND_0) and
 // l = 1 (HPLANE_1_VELOCITY, hplane_1_velocity.F90:79-207: the row-2 forms except
 // INH_V_QZ_INT_1 / INH_V_W_INT_1, inh_v_qz_int_1.F90:49-79, inh_v_w_int_1.F90:51-82).
-__device__ __forceinline__ void velocity_cell(const KParams &k, const int i, const int l) {
+// The new QX, U, QZ, W of cell (i,l) into o[0..3], PML memories updated in
+// place when ST (a recompute of a neighbour's cell leaves them alone). OUT:
+// o also holds, for a field the cell does not update, its value in the set
+// being written (which no kernel writes this step).
+template <bool ST, bool OUT>
+__device__ __forceinline__ void velocity_cell_t(const KParams &k, const int i, const int l, float *o) {
   const int T = k.T, MX = k.MX, NX = k.NX;
   const int ix = i - k.PMLL;
   const bool pmlrow = (l >= k.MZ - 1);
@@ -309,10 +315,12 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
           qn = v0.x * q - v0.y * (oa * (od * dxx + g) + tz1) - v0.z * (oa * (od * pxx + hh));
           un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + tz1) + v1.y * (oa * (od * pxx + s));
         }
-        G[mo] = ob * g + oc * dxx;
-        H[mo] = ob * hh + oc * pxx;
-        R[mo] = ob * r + oc * dxx;
-        S[mo] = ob * s + oc * pxx;
+        if (ST) {
+          G[mo] = ob * g + oc * dxx;
+          H[mo] = ob * hh + oc * pxx;
+          R[mo] = ob * r + oc * dxx;
+          S[mo] = ob * s + oc * pxx;
+        }
       }
     } else {
       // INH_QX_INT_2ND_PML (inh_qx_int_2nd_pml.F90:60-143), INH_V_U_INT_2ND_PML
@@ -337,16 +345,23 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         const float g = G[mo], hh = H[mo], r = R[mo], s = S[mo];
         qn = v0.x * q - v0.y * (oa * (od * dxx + g)) - v0.y * (za * (zd * dxz + gz)) - v0.z * (oa * (od * pxx + hh));
         un = u + v0.w * q + v1.x * (oa * (od * dxx + r) + za * (zd * dxz + rz)) + v1.y * (oa * (od * pxx + s));
-        G[mo] = ob * g + oc * dxx;
-        H[mo] = ob * hh + oc * pxx;
-        R[mo] = ob * r + oc * dxx;
-        S[mo] = ob * s + oc * pxx;
+        if (ST) {
+          G[mo] = ob * g + oc * dxx;
+          H[mo] = ob * hh + oc * pxx;
+          R[mo] = ob * r + oc * dxx;
+          S[mo] = ob * s + oc * pxx;
+        }
       }
-      k.GXZH[zo] = zb * gz + zc * dxz;
-      k.RXZH[zo] = zb * rz + zc * dxz;
+      if (ST) {
+        k.GXZH[zo] = zb * gz + zc * dxz;
+        k.RXZH[zo] = zb * rz + zc * dxz;
+      }
     }
-    FA(k.QX1, i, l) = qn;
-    FA(k.U1, i, l) = un;
+    o[0] = qn;
+    o[1] = un;
+  } else if (OUT) {
+    o[0] = FA(k.QX1, i, l);
+    o[1] = FA(k.U1, i, l);
   }
 
   // ---- QZ and W, x in [2-T, MX-1+T]
@@ -398,8 +413,10 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         const float pzz = FA(k.PR1, i, l) - FA(k.PR1, i, l - 1);
         qn = v1.z * q - v1.w * (dzz + oa * (od * dxz + g)) - v2.x * pzz;
         wn = w + v2.y * q + v2.z * (dzz + oa * (od * dxz + r)) + v2.w * pzz;
-        G[mo] = ob * g + oc * dxz;
-        R[mo] = ob * r + oc * dxz;
+        if (ST) {
+          G[mo] = ob * g + oc * dxz;
+          R[mo] = ob * r + oc * dxz;
+        }
       }
     } else {
       // INH_QZ_INT_2ND_PML (inh_qz_int_2nd_pml.F90:57-144), INH_V_W_INT_2ND_PML
@@ -423,16 +440,38 @@ __device__ __forceinline__ void velocity_cell(const KParams &k, const int i, con
         const float g = G[mo], r = R[mo];
         qn = v1.z * q - v1.w * (za * (zd * dzz + gz)) - v1.w * (oa * (od * dxz + g)) - v2.x * (za * (zd * pzz + hz));
         wn = w + v2.y * q + v2.z * (za * (zd * dzz + rz) + oa * (od * dxz + r)) + v2.w * (za * (zd * pzz + sz));
-        G[mo] = ob * g + oc * dxz;
-        R[mo] = ob * r + oc * dxz;
+        if (ST) {
+          G[mo] = ob * g + oc * dxz;
+          R[mo] = ob * r + oc * dxz;
+        }
       }
-      k.GZZH[zo] = zb * gz + zc * dzz;
-      k.HZZH[zo] = zb * hz + zc * pzz;
-      k.RZZH[zo] = zb * rz + zc * dzz;
-      k.SZZH[zo] = zb * sz + zc * pzz;
+      if (ST) {
+        k.GZZH[zo] = zb * gz + zc * dzz;
+        k.HZZH[zo] = zb * hz + zc * pzz;
+        k.RZZH[zo] = zb * rz + zc * dzz;
+        k.SZZH[zo] = zb * sz + zc * pzz;
+      }
     }
-    FA(k.QZ1, i, l) = qn;
-    FA(k.W1, i, l) = wn;
+    o[2] = qn;
+    o[3] = wn;
+  } else if (OUT) {
+    o[2] = FA(k.QZ1, i, l);
+    o[3] = FA(k.W1, i, l);
+  }
+}
+
+// The velocity half-step of cell (i,l): the new values stored, the PML
+// memories updated.
+__device__ __forceinline__ void velocity_cell(const KParams &k, const int i, const int l) {
+  float o[4];
+  velocity_cell_t<true, false>(k, i, l, o);
+  if (i >= 3 - k.T && i <= k.MX - 1 + k.T) {
+    FA(k.QX1, i, l) = o[0];
+    FA(k.U1, i, l) = o[1];
+  }
+  if (i >= 2 - k.T && i <= k.MX - 1 + k.T) {
+    FA(k.QZ1, i, l) = o[2];
+    FA(k.W1, i, l) = o[3];
   }
 }
 
@@ -1093,6 +1132,172 @@ __device__ __forceinline__ void edges_block(const KParams &k, const int bx, floa
       }
     }
   }
+}
+
+// ------------------------------------------------ edge rows with the velocities
+// The velocity half-step's edge blocks (Split::fold), kBlock = 2 * kEdgeCols
+// threads: thread t takes column c + t % kEdgeCols of the block's first or
+// second row (t / kEdgeCols).
+static_assert(kBlock == 2 * kEdgeCols, "an edge block is two rows of kEdgeCols columns");
+constexpr int kEdgeLds = 4 * (kEdgeCols + 2) + 2 * kEdgeCols + 2 * (kEdgeCols + 1);
+
+// Top block bx: rows 2 and 3 of its columns [c, ce), then row 1 as
+// HPLANE_1_VELOCITY forms it (hplane_1_velocity.F90:61-76: INH_V_QX_INT_1_SI,
+// inh_v_qx_int_1_si.F90:26-31, for all x, then INH_W_QZ_INT_1_SI,
+// inh_w_qz_int_1_si.F90:30-44, from the new rows 1-3). Row 1 at column i
+// needs the new row 2 at i-1 .. i+1: the block recomputes row 2 at c-1 and
+// ce (interior cells, no PML memory) without storing them.
+__device__ __forceinline__ void top_block(const KParams &k, const Split &sp, int bx, float *lds) {
+  constexpr int E = kEdgeCols;
+  const int T = k.T, MX = k.MX, PX1 = k.PMLL + k.NX;
+  const int last = sp.etop - 1;
+  const int c = bx == last ? sp.elast : k.PMLL + E * bx;
+  const int ce = bx == last ? PX1 : min(k.PMLL + E * (bx + 1), sp.elast);
+  float *sQ2 = lds, *sU2 = lds + (E + 2), *sZ2 = lds + 2 * (E + 2), *sW2 = lds + 3 * (E + 2);
+  float *sZ3 = lds + 4 * (E + 2), *sW3 = sZ3 + E, *su = sW3 + E, *sq = su + (E + 1);
+  const int t = threadIdx.x, r = t / E, q = t % E;
+  const int i = c + q;
+  if (i < ce) {
+    float o[4];
+    velocity_cell_t<true, true>(k, i, 2 + r, o);
+    if (i >= 3 - T && i <= MX - 1 + T) {
+      FA(k.QX1, i, 2 + r) = o[0];
+      FA(k.U1, i, 2 + r) = o[1];
+    }
+    if (i >= 2 - T && i <= MX - 1 + T) {
+      FA(k.QZ1, i, 2 + r) = o[2];
+      FA(k.W1, i, 2 + r) = o[3];
+    }
+    if (r == 0) {
+      sQ2[q + 1] = o[0]; sU2[q + 1] = o[1]; sZ2[q + 1] = o[2]; sW2[q + 1] = o[3];
+    } else {
+      sZ3[q] = o[2]; sW3[q] = o[3];
+    }
+  }
+  // row 2 of the neighbours' edge columns (second wave: its first two lanes)
+  if (t == E || t == E + 1) {
+    const int h = t == E ? c - 1 : ce;
+    if (h >= k.PMLL && h < PX1) {
+      float o[4];
+      velocity_cell_t<false, true>(k, h, 2, o);
+      const int qh = h - c + 1;
+      sQ2[qh] = o[0]; sU2[qh] = o[1]; sZ2[qh] = o[2]; sW2[qh] = o[3];
+    }
+  }
+  __syncthreads();
+  // row 1, U and QX: columns c .. ce (ce for the next column's W and QZ)
+  const int w = ce - c;
+  if (t <= w) {
+    const int j = c + t;
+    if (j < PX1 && j >= 3 - T && j <= MX - 1 + T) {
+      const float u1 = sU2[t + 1] + sW2[t + 1] - sW2[t];
+      const float q1 = sQ2[t + 1] + sZ2[t + 1] - sZ2[t];
+      su[t] = u1;
+      sq[t] = q1;
+      if (t < w) {
+        FA(k.U1, j, 1) = u1;
+        FA(k.QX1, j, 1) = q1;
+      }
+    }
+  }
+  __syncthreads();
+  // row 1, W and QZ
+  if (t < w && i >= 3 - T && i <= MX - 2 + T) {
+    const int jm = k.JM[i + k.OFF] - 1;  // JM(I,0)
+    const float2 aux = k.atab[jm];
+    const float u0 = su[t], u1 = su[t + 1];
+    const float q0 = sq[t], q1 = sq[t + 1];
+    const float du = u1 - u0 + sU2[t + 2] - sU2[t + 1];
+    FA(k.W1, i, 1) = sW3[t] + aux.x * du;
+    FA(k.QZ1, i, 1) = sZ3[t] + aux.y * du + (q1 - q0 + sQ2[t + 2] - sQ2[t + 1]);
+  }
+}
+
+// Bottom block bx: rows PMLZH-2 and PMLZH-1 of its columns, then row PMLZH
+// (HPLANE_PMLZH0_VELOCITY, hplane_pmlzh0_velocity.F90:34-63) from them and the
+// two stored levels, and the history advanced as the store of
+// time_loop.F90:133-155 sees it next step.
+__device__ __forceinline__ void bottom_block(const KParams &k, int bx, float *lds) {
+  constexpr int E = kEdgeCols;
+  const int T = k.T, MX = k.MX, NX = k.NX, lst = k.PMLZH;
+  const int t = threadIdx.x, r = t / E, q = t % E;
+  const int ix = bx * E + q;
+  const int i = ix + k.PMLL;
+  const bool col = ix < NX;
+  // the history and row PMLZH (nothing else writes them this step)
+  float* const F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
+  float bf0[4] = {0, 0, 0, 0}, bsc[4][3] = {}, bsp[4][3] = {};
+  if (col && r == 0) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+      const float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+      bf0[f] = FA(F[f], i, lst);
+      bsc[f][0] = sc[0]; bsc[f][1] = sc[1]; bsc[f][2] = sc[2];
+      bsp[f][0] = sp[0]; bsp[f][1] = sp[1]; bsp[f][2] = sp[2];
+    }
+  }
+  if (col) {
+    float o[4];
+    velocity_cell_t<true, true>(k, i, lst - 2 + r, o);
+    if (i >= 3 - T && i <= MX - 1 + T) {
+      FA(k.QX1, i, lst - 2 + r) = o[0];
+      FA(k.U1, i, lst - 2 + r) = o[1];
+    }
+    if (i >= 2 - T && i <= MX - 1 + T) {
+      FA(k.QZ1, i, lst - 2 + r) = o[2];
+      FA(k.W1, i, lst - 2 + r) = o[3];
+    }
+    // lds[(f * 2 + r) * E + q], fields in the order U, W, QX, QZ
+    lds[(0 * 2 + r) * E + q] = o[1];
+    lds[(1 * 2 + r) * E + q] = o[3];
+    lds[(2 * 2 + r) * E + q] = o[0];
+    lds[(3 * 2 + r) * E + q] = o[2];
+  }
+  __syncthreads();
+  if (!col || r != 0) return;
+  const bool upd = i >= 1 - T && i <= MX + T;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const float *cf = (f == 1 || f == 3) ? k.abw : k.abh;
+    float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
+    float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+    const float f1 = lds[(f * 2 + 1) * E + q], f2 = lds[(f * 2 + 0) * E + q];
+    sp[0] = bsc[f][0]; sp[1] = bsc[f][1]; sp[2] = bsc[f][2];
+    if (upd) {
+      const float n = cf[0] * f1 + cf[1] * f2 + cf[2] * bsc[f][0] + cf[3] * bsc[f][1] +
+                      cf[4] * bsc[f][2] + cf[5] * bsp[f][0] + cf[6] * bsp[f][1] + cf[7] * bsp[f][2];
+      FA(F[f], i, lst) = n;
+      sc[0] = n; sc[1] = f1; sc[2] = f2;
+    } else {
+      sc[0] = bf0[f]; sc[1] = f1; sc[2] = f2;
+    }
+  }
+}
+
+// Block b of a velocity launch over split sp: the edge blocks (fold), then
+// nrest per-cell blocks, then nruns run blocks.
+__device__ __forceinline__ void velocity_block(const KParams &k, const Split &sp, int b, int nrest,
+                                               int nruns, float *lds) {
+  if (b < sp.etop) {
+    top_block(k, sp, b, lds);
+    return;
+  }
+  b -= sp.etop;
+  if (b < sp.ebot) {
+    bottom_block(k, b, lds);
+    return;
+  }
+  b -= sp.ebot;
+  if (b < nrest) {
+    int i, l;
+    if (rest_cell(sp, b * kBlock + threadIdx.x, &i, &l)) velocity_cell(k, i, l);
+    return;
+  }
+  int tile;
+  const int r = run_rect(sp, xcd_tile(b - nrest, nruns), &tile);
+  velocity_run<kRun>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+                     threadIdx.x & 63);
 }
 
 }  // namespace bqk

@@ -199,7 +199,7 @@ def run_batched(args):
         "autotune_ms_per_step": {f"{r}/{['grid', 'persistent'][sc]}": round(v, 4)
                                  for (r, sc), v in tune.items()},
         "gcell_per_s": round(gcell, 2),
-        "roofline": {"bound": "hbm", "kernel": "whole batched step (kb_step, kb_velocity, kb_edges)",
+        "roofline": {"bound": "hbm", "kernel": "whole batched step (kb_step, kb_rest, kb_velocity with the edge rows)",
                      "basis": "104 B per cell-step over all shots' cells",
                      "algorithmic_bytes": int(B_STEP * step_cells),
                      "achieved": round(gcell * B_STEP, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

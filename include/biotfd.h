@@ -244,7 +244,10 @@ int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
  * HIP events on the handle's stream: out[0] k_step (both half-steps of the
  * interior cells, the stress half-step of all other cells and the previous
  * step's receiver samples), out[1] k_velocity (velocity half-step of the other
- * cells), out[2] k_edges (rows 1 and PMLZH). Timing must have been enabled
+ * cells, with rows 1 and PMLZH where its edge blocks take them), out[2]
+ * k_edges (rows 1 and PMLZH; 0 where k_velocity takes them: every grid with
+ * TPML <= 124 and at least 3 rows between the surface and bottom edge blocks,
+ * in single steps). Timing must have been enabled
  * with bq_enable_timing before that run: every = n >= 1 brackets the launches
  * of every n-th step of a run (its first step included) with events, 0 turns
  * timing off. */

@@ -111,6 +111,33 @@ static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) 
   };
   long long ns = 0, nv = 0;
   if (runs(p.ss, cs, 5, &ns) || runs(p.sv, cv, 2, &nv)) return 1;
+  // edge blocks of the velocity half-step: top blocks over rows 2, 3 (stress
+  // imaging), bottom blocks over rows PMLZH-2, PMLZH-1, all columns; the
+  // columns each top block recomputes for its neighbours (one each side of
+  // every boundary between blocks) must use the interior formulas, [4, MX-2]
+  const Split &sv = p.sv;
+  if (p.ss.fold || p.ss.etop || p.ss.ebot) return fail("stress split with edge blocks", MX, MZ, T, 0, 0);
+  if (sv.fold) {
+    const int PX1 = g.PMLXH + 1;
+    if ((sv.etop > 0) != (zv == 2)) return fail("top blocks", MX, MZ, T, 0, 0);
+    int x = g.PMLL;
+    for (int b = 0; b < sv.etop; ++b) {
+      const int c = b == sv.etop - 1 ? sv.elast : g.PMLL + kEdgeCols * b;
+      const int ce = b == sv.etop - 1 ? PX1 : std::min(g.PMLL + kEdgeCols * (b + 1), sv.elast);
+      if (c != x || ce <= c || ce - c > kEdgeCols) return fail("top block columns", MX, MZ, T, c, 2);
+      if (b > 0 && (c - 1 < 4 || c > MX - 2)) return fail("top block halo", MX, MZ, T, c, 2);
+      for (int z = 2; z <= 3; ++z)
+        for (int xx = c; xx < ce; ++xx) { ++at(cv, xx, z); ++nv; }
+      x = ce;
+    }
+    if (sv.etop && x != PX1) return fail("top blocks end", MX, MZ, T, x, 2);
+    if (sv.ebot != (g.NX + kEdgeCols - 1) / kEdgeCols || g.PMLZH - 2 < (sv.etop ? 4 : 2))
+      return fail("bottom blocks", MX, MZ, T, 0, g.PMLZH);
+    for (int z = g.PMLZH - 2; z <= g.PMLZH - 1; ++z)
+      for (int xx = g.PMLL; xx < PX1; ++xx) { ++at(cv, xx, z); ++nv; }
+  } else if (sv.etop || sv.ebot) {
+    return fail("edge blocks without fold", MX, MZ, T, 0, 0);
+  }
   if (ns != p.stress_cells || nv != p.velocity_cells) return fail("cell counts", MX, MZ, T, 0, 0);
   for (int z = 0; z <= g.PMLZH; ++z)
     for (int x = g.PMLL; x <= g.PMLXH; ++x) {
