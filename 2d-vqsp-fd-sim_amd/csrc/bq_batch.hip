@@ -56,7 +56,9 @@ struct ShotDev {
 };
 
 // Prefix sums over the shots of each launch's blocks ([nshots + 1] each).
-enum { C_FUSED = 0, C_OTHER, C_VEL, C_EDGE, C_SAMPLE, C_LB, C_LBX, C_N };
+// C_FUSED: all fused blocks of a shot (persistent schedule); C_F1, C_F2, C_F4:
+// its full-tile, half-tile and quarter-tile blocks (grid schedule).
+enum { C_FUSED = 0, C_OTHER, C_VEL, C_EDGE, C_SAMPLE, C_LB, C_LBX, C_F1, C_F2, C_F4, C_N };
 
 struct BatchArgs {
   const ShotDev *shots;
@@ -78,6 +80,40 @@ __device__ __forceinline__ int find_shot(const int *cum, int n, int b) {
 }
 
 __device__ __forceinline__ void other_item(const BatchArgs &B, int w, int kofs);
+
+// Blocks of a grid-schedule kb_step launch: every XCD (block b runs on XCD
+// b % 8) takes its share of the full-tile blocks of all shots, then of the
+// half-tile blocks, then of the quarter-tile blocks, so the short blocks are
+// dispatched last everywhere and fill in behind the long ones.
+inline int batch_fused_launch(const int *total) {
+  int l = 0;
+  for (int c = C_F1; c <= C_F4; ++c) l += (total[c] + 7) / 8;
+  return 8 * l;
+}
+
+// Shot and fused block (numbered as fused_block takes them) of grid-schedule
+// block b; false past the XCD's share.
+__device__ __forceinline__ bool batch_fused(const BatchArgs &B, int b, int *shot, int *blk) {
+  const int x = b & 7;
+  int q = b >> 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int n = B.total[C_F1 + c], per = n >> 3, rem = n & 7;
+    const int cnt = per + (x < rem);
+    if (q < cnt) {
+      const int w = x * per + min(x, rem) + q;
+      const int *cum = B.cum + (C_F1 + c) * (B.nshots + 1);
+      const int s = find_shot(cum, B.nshots, w);
+      const Fused &fu = B.shots[s].fu;
+      const int t = w - cum[s];
+      *shot = s;
+      *blk = c == 0 ? t : fu.nfx * fu.nfz + (c == 2 ? fu.nb2 : 0) + t;
+      return true;
+    }
+    q -= cnt;
+  }
+  return false;
+}
 
 // Item w of kb_step: items [0, total[C_FUSED]) are fused tiles (w is the tile
 // number, tiles numbered shot after shot, row-major within a shot); the rest
@@ -167,10 +203,13 @@ __global__ __launch_bounds__(kStepBlock, 4) void kb_step(BatchArgs B, int kl) {
   const int kofs = *B.dstep + kl;
   const int total = B.total[C_FUSED] + B.total[C_OTHER];
   if (!PERSIST) {
-    // block b runs on XCD b % 8, which takes a contiguous band of tiles
-    const int nf = B.total[C_FUSED];
-    const int b = blockIdx.x;
-    step_item<FR>(B, b < nf ? xcd_tile(b, nf) : b, kofs, L);
+    int s, blk;
+    if (batch_fused(B, blockIdx.x, &s, &blk)) {
+      const ShotDev &sd = B.shots[s];
+      const KParams k = sd.kp[(sd.base_par + kofs) & 1];   // a copy: held in registers
+      const Fused fu = sd.fu;
+      fused_block<FR>(k, fu, blk, sd.base_step + kofs, L);
+    }
     return;
   }
   const int xcd = blockIdx.x & 7;
@@ -193,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void kb_rest(BatchArgs B, int kl) {
 }
 
 template <bool PERSIST>
-__global__ __launch_bounds__(kBlock, 5) void kb_velocity(BatchArgs B, int kl) {
+__global__ __launch_bounds__(kBlock) void kb_velocity(BatchArgs B, int kl) {
   __shared__ int item;
   __shared__ float lds[kEdgeLds];
   const int kofs = *B.dstep + kl;
@@ -303,14 +342,14 @@ struct bq_batch {
   ShotDev *dshots = nullptr;
   int *dcum = nullptr;     // [C_N][nshots + 1]
   int *dctl = nullptr;     // [0]: step counter, [8..19]: persistent queue
-  int total[C_N] = {0, 0, 0, 0, 0, 0, 0};
+  int total[C_N] = {};
   int fr = 128;            // fused tile height
   int sched = BQ_SCHED_GRID;
   int use_graph = 1;
   int cus = 256;
   hipGraphExec_t graph = nullptr;
   int graph_fr = 0, graph_sched = -1;
-  int graph_total[C_N] = {-1, -1, -1, -1, -1, -1, -1};   // block counts the graph was made for
+  int graph_total[C_N] = {};   // block counts the graph was made for (none: graph null)
   hipEvent_t ev[2] = {nullptr, nullptr};
   int last_steps = 0;
   std::vector<double> tune_ms;   // per candidate, ms per step
@@ -365,8 +404,9 @@ int layout(bq_batch *b, int fr) {
     d.lb_on = h->lb != nullptr;
     d.nlb = d.lb_on ? 2 * lb_tiles(lb_lz(h->g.MZ)) + lb_tiles(lb_lx(h->g.MX)) : 0;
     d.nlbx = d.lb_on ? (int)((lb_band_size(h->g.MX, h->g.MZ) + kBlock - 1) / kBlock) : 0;
-    const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample, h->sv.etop + h->sv.ebot + d.vrest + d.vruns, d.nedge,
-                          d.nsample, d.nlb, d.nlbx};
+    const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample,
+                          h->sv.etop + h->sv.ebot + d.vrest + d.vruns, d.nedge, d.nsample, d.nlb, d.nlbx,
+                          d.fu.nfx * d.fu.nfz, d.fu.nb2, d.fu.nb4};
     for (int c = 0; c < C_N; ++c) cum[(size_t)c * (n + 1) + s + 1] = cum[(size_t)c * (n + 1) + s] + cnt[c];
   }
   for (int c = 0; c < C_N; ++c) b->total[c] = cum[(size_t)c * (n + 1) + n];
@@ -410,8 +450,8 @@ void launch_step_fr(bq_batch *b, const BatchArgs &a, int kl, int advance) {
                          dim3(kBlock), 0, b->stream, a, kl);
   } else {
     if (a.total[C_FUSED])
-      hipLaunchKernelGGL((kb_step<FR, false>), dim3(a.total[C_FUSED]), dim3(kStepBlock), 0, b->stream,
-                         a, kl);
+      hipLaunchKernelGGL((kb_step<FR, false>), dim3(batch_fused_launch(a.total)), dim3(kStepBlock), 0,
+                         b->stream, a, kl);
     if (a.total[C_OTHER])
       hipLaunchKernelGGL(kb_rest, dim3(a.total[C_OTHER]), dim3(kBlock), 0, b->stream, a, kl);
     if (nvel) hipLaunchKernelGGL((kb_velocity<false>), dim3(nvel), dim3(kBlock), 0, b->stream, a, kl);
