@@ -129,3 +129,6 @@ GPU_EXTRA["narrow_nosi"] = lambda: narrow_tiles(stress_imaging=False)
 # 1000 columns: the quarter tiles [791, 919) are followed by a second quarter
 # column masked to its first 78 columns, [919, 997) up to MX-3
 GPU_EXTRA["narrow_masked"] = lambda: narrow_tiles(mx=1000)
+# a PML too wide for the velocity launch's edge blocks (TPML > 124): rows 1 and
+# PMLZH in a launch of their own
+GPU_EXTRA["wide_pml"] = lambda: tiny_layered(steps=60, tpml=130)

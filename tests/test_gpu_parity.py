@@ -103,18 +103,22 @@ def test_bitexact_vs_reference_snapshots(name):
 
 
 @pytest.mark.parametrize("every", [1, 3])
-def test_kernel_timing_leaves_results_unchanged(every):
+@pytest.mark.parametrize("tpml", [8, 130])
+def test_kernel_timing_leaves_results_unchanged(every, tpml):
     """Timed runs (events around every n-th step) give the untimed fields and
-    positive per-launch durations."""
-    job = jobs.tiny_layered()
-    print(f"timing every {every}", flush=True)
+    positive per-launch durations: k_edges is 0 where the velocity launch takes
+    the edge rows (TPML 8) and positive where it runs on its own (TPML 130,
+    too wide for the velocity launch's edge blocks)."""
+    job = jobs.tiny_layered(tpml=tpml)
+    print(f"timing every {every}, TPML {tpml}", flush=True)
     a, b = lib.Simulation(job), lib.Simulation(job)
     a.enable_timing(every)
     a.run(10)
     b.run(10)
     t = a.kernel_times()
     assert set(t) == {"k_step", "k_velocity", "k_edges"}
-    assert all(v > 0 for v in t.values()), t
+    assert t["k_step"] > 0 and t["k_velocity"] > 0, t
+    assert (t["k_edges"] > 0) == (tpml > 124), t
     for f in FIELDS:
         np.testing.assert_array_equal(a.field(f), b.field(f))
     a.close()
