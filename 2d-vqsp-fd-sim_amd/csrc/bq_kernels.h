@@ -1224,19 +1224,7 @@ __device__ __forceinline__ void bottom_block(const KParams &k, int bx, float *ld
   const int ix = bx * E + q;
   const int i = ix + k.PMLL;
   const bool col = ix < NX;
-  // the history and row PMLZH (nothing else writes them this step)
   float* const F[4] = {k.U1, k.W1, k.QX1, k.QZ1};
-  float bf0[4] = {0, 0, 0, 0}, bsc[4][3] = {}, bsp[4][3] = {};
-  if (col && r == 0) {
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      const float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
-      const float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
-      bf0[f] = FA(F[f], i, lst);
-      bsc[f][0] = sc[0]; bsc[f][1] = sc[1]; bsc[f][2] = sc[2];
-      bsp[f][0] = sp[0]; bsp[f][1] = sp[1]; bsp[f][2] = sp[2];
-    }
-  }
   if (col) {
     float o[4];
     velocity_cell_t<true, true>(k, i, lst - 2 + r, o);
@@ -1256,21 +1244,23 @@ __device__ __forceinline__ void bottom_block(const KParams &k, int bx, float *ld
   }
   __syncthreads();
   if (!col || r != 0) return;
+  // the history and row PMLZH (nothing else writes them this step)
   const bool upd = i >= 1 - T && i <= MX + T;
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     const float *cf = (f == 1 || f == 3) ? k.abw : k.abh;
     float *sc = k.SZc + ((size_t)f * NX + ix) * 3;
     float *sp = k.SZp + ((size_t)f * NX + ix) * 3;
+    const float c0 = sc[0], c1 = sc[1], c2 = sc[2], p0 = sp[0], p1 = sp[1], p2 = sp[2];
     const float f1 = lds[(f * 2 + 1) * E + q], f2 = lds[(f * 2 + 0) * E + q];
-    sp[0] = bsc[f][0]; sp[1] = bsc[f][1]; sp[2] = bsc[f][2];
+    sp[0] = c0; sp[1] = c1; sp[2] = c2;
     if (upd) {
-      const float n = cf[0] * f1 + cf[1] * f2 + cf[2] * bsc[f][0] + cf[3] * bsc[f][1] +
-                      cf[4] * bsc[f][2] + cf[5] * bsp[f][0] + cf[6] * bsp[f][1] + cf[7] * bsp[f][2];
+      const float n = cf[0] * f1 + cf[1] * f2 + cf[2] * c0 + cf[3] * c1 + cf[4] * c2 + cf[5] * p0 +
+                      cf[6] * p1 + cf[7] * p2;
       FA(F[f], i, lst) = n;
       sc[0] = n; sc[1] = f1; sc[2] = f2;
     } else {
-      sc[0] = bf0[f]; sc[1] = f1; sc[2] = f2;
+      sc[0] = FA(F[f], i, lst); sc[1] = f1; sc[2] = f2;
     }
   }
 }
