@@ -251,9 +251,13 @@ int bq_create(const bq_desc *d, bq_handle **out) {
     *f0[f] = h->fields[0] + f * fsz;
     *f1[f] = h->fields[1] + f * fsz;
   }
+  // the material index in 16 bits where the table allows (2 B per cell
+  // instead of 4 on every step's reads)
   int *jmd = nullptr;
   rc |= dalloc(h, &jmd, fsz);
-  k.JM = jmd;
+  const bool jm16 = g.JMNUM <= 65535;
+  k.JM = jm16 ? nullptr : jmd;
+  k.JM16 = jm16 ? reinterpret_cast<const unsigned short *>(jmd) : nullptr;
   float4 *st = nullptr, *vt = nullptr;
   float2 *at = nullptr;
   rc |= dalloc(h, &st, 3 * (size_t)g.JMNUM);
@@ -404,7 +408,14 @@ int bq_upload_model(bq_handle *h, const int32_t *jm, const float *tab) {
     at[2 * m] = t[T_AUXW];
     at[2 * m + 1] = t[T_AUXQZ];
   }
-  HIPCHK(h, hipMemcpyAsync((void *)h->kp.JM, jp.data(), jp.size() * 4, hipMemcpyHostToDevice, h->stream));
+  std::vector<uint16_t> jp16;
+  if (h->kp.JM16) {
+    jp16.assign(jp.begin(), jp.end());   // 1 <= jm <= JMNUM <= 65535, checked above
+    HIPCHK(h, hipMemcpyAsync((void *)h->kp.JM16, jp16.data(), jp16.size() * 2, hipMemcpyHostToDevice,
+                             h->stream));
+  } else {
+    HIPCHK(h, hipMemcpyAsync((void *)h->kp.JM, jp.data(), jp.size() * 4, hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(h, hipMemcpyAsync((void *)h->kp.stab, st.data(), st.size() * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync((void *)h->kp.vtab, vt.data(), vt.size() * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync((void *)h->kp.atab, at.data(), at.size() * 4, hipMemcpyHostToDevice, h->stream));

@@ -101,7 +101,6 @@ __device__ __forceinline__ void fused2_block(const KParams &k, const Fused2 &fu,
   const float *__restrict__ Wf = k.W0;
   const float *__restrict__ QX = k.QX0;
   const float *__restrict__ QZ = k.QZ0;
-  const int *__restrict__ JM = k.JM;
   const int s0 = a - 6, s1 = e + 5;  // march rows
   // prefill: U rows s0-2 .. s0, W and QZ rows s0-1 .. s0+1
   for (int r = s0 - 2; r <= s0; ++r) {
@@ -129,10 +128,10 @@ __device__ __forceinline__ void fused2_block(const KParams &k, const Fused2 &fu,
     pu = U[r + P + gx]; pw = Wf[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
     if (hl) { fhu = U[r + P + hlx]; fhw = Wf[r + 2 * P + hlx]; fhz = QZ[r + 2 * P + hlx]; fhq = QX[r + hlx]; }
     ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
-    fjm = JM[r + gx] - 1;
+    fjm = jm_at(k, r + gx);
     if (h1) {
       fhtxx = k.TXX0[r + h1x]; fhtzz = k.TZZ0[r + h1x]; fhtxz = k.TXZ0[r + h1x];
-      fhpr = k.PR0[r + h1x]; fhjm = JM[r + h1x] - 1;
+      fhpr = k.PR0[r + h1x]; fhjm = jm_at(k, r + h1x);
     }
   };
   fetch(s0);

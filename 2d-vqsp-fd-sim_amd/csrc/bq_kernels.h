@@ -36,7 +36,10 @@ struct KParams {
   // sets trade places after every step
   const float *U0, *W0, *QX0, *QZ0, *TXX0, *TZZ0, *TXZ0, *PR0;
   float *U1, *W1, *QX1, *QZ1, *TXX1, *TZZ1, *TXZ1, *PR1;
+  // material index (1-based), 16-bit where the table has at most 65535
+  // entries (JM16), else 32-bit (JM); the other pointer is null
   const int *JM;
+  const unsigned short *JM16;
   const float4 *stab;  // per material: {MU,SXX1,SXX2,SXX3},{SZZ1,SZZ2,SZZ3,P1},{P2,P3,-,-}
   const float4 *vtab;  // per material: {eQX,R1QX,R2QX,fU},{R1U,R3U,eQZ,R1QZ},{R2QZ,fW,R1W,R3W}
   const float2 *atab;  // per material: {AUXIL_W, AUXIL_QZ}
@@ -60,6 +63,11 @@ struct KParams {
 // consecutive addresses
 #define ZH1(t, j, ix) (k.z1[t][(size_t)(j) * k.NX + (ix)])
 #define ZH2(t, j, ix) (k.z2[t][(size_t)(j) * k.NX + (ix)])
+
+// Material index of cell c (row-major with the field pitch), 0-based.
+__device__ __forceinline__ int jm_at(const KParams &k, size_t c) {
+  return (k.JM16 ? (int)k.JM16[c] : k.JM[c]) - 1;
+}
 
 // One-sided surface stencils of the explicit free surface (binary32 quotients,
 // as the reference's constant expressions fold them).
@@ -91,7 +99,7 @@ __device__ __forceinline__ void stress_cell(const KParams &k, const int i, const
   const bool zpml = zh0 || pmlrow;
   const int LL = zh0 ? T + 1 : (pmlrow ? l - MZ + 1 : 0);
   const float C = k.CH, DT = k.DT;
-  const int jm = k.JM[(size_t)l * k.P + ix] - 1;
+  const int jm = jm_at(k, (size_t)l * k.P + ix);
   const float4 s0 = k.stab[3 * jm];
 
   // ---- shear strain and TXZ: STRAIN_UW_XZ (strain_uw_xz.F90:61-103) or
@@ -268,7 +276,7 @@ __device__ __forceinline__ void velocity_cell_t(const KParams &k, const int i, c
   const bool pmlrow = (l >= k.MZ - 1);
   const int LL = l - k.MZ + 1;
   const float A = k.A, B = k.B;
-  const int jm = k.JM[(size_t)l * k.P + ix] - 1;
+  const int jm = jm_at(k, (size_t)l * k.P + ix);
   const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
   // v0 = {eQX, R1QX, R2QX, fU}, v1 = {R1U, R3U, eQZ, R1QZ}, v2 = {R2QZ, fW, R1W, R3W}
 
@@ -535,7 +543,6 @@ __device__ __forceinline__ void stress_run(const KParams &k, int x0, int xl, int
   const float *__restrict__ W = k.W0;
   const float *__restrict__ QX = k.QX0;
   const float *__restrict__ QZ = k.QZ0;
-  const int *__restrict__ JM = k.JM;
   const float *__restrict__ TXX = k.TXX0;
   const float *__restrict__ TZZ = k.TZZ0;
   const float *__restrict__ TXZ = k.TXZ0;
@@ -565,7 +572,7 @@ __device__ __forceinline__ void stress_run(const KParams &k, int x0, int xl, int
     const XNb un = xneighbours(u[j + 2], U[r + hx], lane);
     const XNb wn = xneighbours(w[j + 1], W[r + hx], lane);
     const XNb qn = xneighbours(qx0, QX[r + hx], lane);
-    const int jm = JM[r + ix] - 1;
+    const int jm = jm_at(k, r + ix);
     const float4 s0 = k.stab[3 * jm], s1 = k.stab[3 * jm + 1], s2 = k.stab[3 * jm + 2];
     const float um2 = u[j], um1 = u[j + 1], u0 = u[j + 2], up1 = u[j + 3];
     const float wm1 = w[j], w0 = w[j + 1], wp1 = w[j + 2], wp2 = w[j + 3];
@@ -612,7 +619,6 @@ __device__ __forceinline__ void velocity_run(const KParams &k, int x0, int xl, i
   const float *__restrict__ TZZ = k.TZZ1;
   const float *__restrict__ TXZ = k.TXZ1;
   const float *__restrict__ PR = k.PR1;
-  const int *__restrict__ JM = k.JM;
   const float *__restrict__ U = k.U0;
   const float *__restrict__ W = k.W0;
   const float *__restrict__ QX = k.QX0;
@@ -644,7 +650,7 @@ __device__ __forceinline__ void velocity_run(const KParams &k, int x0, int xl, i
     const XNb xx = xneighbours(txx0, TXX[r + hx], lane);
     const XNb xzn = xneighbours(xz0, TXZ[r + hx], lane);
     const XNb pn = xneighbours(p0, PR[r + hx], lane);
-    const int jm = JM[r + ix] - 1;
+    const int jm = jm_at(k, r + ix);
     const float4 v0 = k.vtab[3 * jm], v1 = k.vtab[3 * jm + 1], v2 = k.vtab[3 * jm + 2];
     if (!st) continue;
     const float q = QX[r + ix], u = U[r + ix], qz = QZ[r + ix], w = W[r + ix];
@@ -849,7 +855,6 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int
   const float *__restrict__ Wv = k.W0;
   const float *__restrict__ QX = k.QX0;
   const float *__restrict__ QZ = k.QZ0;
-  const int *__restrict__ JM = k.JM;
   // prefill the rings: U rows a-4..a-2, W and QZ rows a-3..a-1
   if (live)
     for (int r = a - 4; r <= a - 2; ++r) {
@@ -876,10 +881,10 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int
     pu = U[r + P + gx]; pw = Wv[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
     if (hl) { fhu = U[r + P + hgx]; fhw = Wv[r + 2 * P + hgx]; fhz = QZ[r + 2 * P + hgx]; fhq = QX[r + hgx]; }
     ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
-    fjm = JM[r + gx] - 1;
+    fjm = jm_at(k, r + gx);
     if (hs) {
       fhtxx = k.TXX0[r + hgs]; fhtzz = k.TZZ0[r + hgs]; fhtxz = k.TXZ0[r + hgs];
-      fhpr = k.PR0[r + hgs]; fhjm = JM[r + hgs] - 1;
+      fhpr = k.PR0[r + hgs]; fhjm = jm_at(k, r + hgs);
     }
   };
   if (live) fetch(a - 2);
@@ -1083,7 +1088,7 @@ __device__ __forceinline__ void edges_block(const KParams &k, const int bx, floa
     up2 = FA(k.U1, i + 1, 2); u2 = FA(k.U1, i, 2);
     w3 = FA(k.W1, i, 3); qz3 = FA(k.QZ1, i, 3);
     qxp2 = FA(k.QX1, i + 1, 2); qx2 = FA(k.QX1, i, 2);
-    aux = k.atab[k.JM[ix] - 1];  // JM(I,0)
+    aux = k.atab[jm_at(k, ix)];  // JM(I,0)
   }
   // phase 1: U(i,1), QX(i,1) for the block's columns and one halo column
   // (stress imaging only; without it row 1 is a per-cell row of k_velocity)
@@ -1203,7 +1208,7 @@ __device__ __forceinline__ void top_block(const KParams &k, const Split &sp, int
   __syncthreads();
   // row 1, W and QZ
   if (t < w && i >= 3 - T && i <= MX - 2 + T) {
-    const int jm = k.JM[i + k.OFF] - 1;  // JM(I,0)
+    const int jm = jm_at(k, i + k.OFF);  // JM(I,0)
     const float2 aux = k.atab[jm];
     const float u0 = su[t], u1 = su[t + 1];
     const float q0 = sq[t], q1 = sq[t + 1];
