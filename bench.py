@@ -38,8 +38,10 @@ B_STEP = 104
 B_HALF = 52
 # What the fused interior kernel must move at least for one cell-step: 8 fields
 # read once, 8 written once, 1 index (the other group's fields are consumed in
-# registers and LDS within the same sweep).
+# registers and LDS within the same sweep). The index is stored in 16 bits
+# where the table has at most 65535 materials.
 B_FUSED_MIN = 68
+B_FUSED_MIN_IDX16 = 66
 # Per-cell materials (every cell its own material): the index is replaced by
 # the 22 precombined coefficients a cell's update needs, 10 for the stress
 # half-step and 12 for the velocity half-step: 88 + 96 = 184 B per cell-step.
@@ -420,6 +422,8 @@ def main():
     b_step, b_half, b_min = ((B_STEP_CELL, B_HALF_CELL, B_FUSED_MIN_CELL) if per_cell
                              else (B_STEP, B_HALF, B_FUSED_MIN))
     job = make_workload(total, args.mx, args.mz, per_cell)
+    if not per_cell and job.table.shape[0] <= 65535:
+        b_min = B_FUSED_MIN_IDX16
     nz, nx = hostsetup.grid_shape(job)
     print(f"[bench] model ready: {nx}x{nz} cells, JMNUM={job.table.shape[0]} "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
