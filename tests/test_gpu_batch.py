@@ -2,8 +2,10 @@
 bit for bit (needs a GPU).
 
 The shots differ in grid size, PML width, surface treatment, bottom boundary,
-source position and receivers; some have fused interior tiles, some only the
-per-cell and wavefront paths. Every schedule - tile height 64 / 128 / 256,
+source position and receivers; some have fused interior tiles (full, half and
+masked quarter tiles), some only the per-cell and wavefront paths; one has a
+PML too wide for the velocity launch's edge blocks, so its edge rows take the
+kb_edges launch beside the others'. Every schedule - tile height 64 / 128 / 256,
 one block per item or persistent, with and without the HIP graph - must give
 each shot exactly the fields and traces of its own run.
 """
@@ -35,6 +37,8 @@ def _shot_jobs():
         jobs.tiny_layered_nosi(steps=STEPS, variant=3),    # explicit surface, AFDA options
         _ktbo_job(),                                       # bottom boundary key 4
         jobs.wide_deep(steps=STEPS, stress_imaging=False),
+        jobs.narrow_tiles(steps=STEPS, mx=1000),           # half and masked quarter tiles
+        jobs.tiny_layered(steps=STEPS, tpml=130),          # edge rows in kb_edges
     ]
 
 
