@@ -49,9 +49,11 @@ __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nr
 }
 
 // First launch of a step: fused interior blocks (long-running, first), then
-// the stress half-step of every other cell (per-cell blocks and runs), then
+// the stress half-step of every other cell (per-cell items and runs), then
 // the receiver samples of the previous step, taken from the current set,
-// which this launch only reads. All parts are independent of each other.
+// which this launch only reads; the items after the fused blocks take
+// kBlock threads each, two to a block. All parts are independent of each
+// other.
 struct Launch1 {
   int nfused, nrest, nruns, nsample, sample_step;
 };
@@ -66,24 +68,24 @@ __global__ __launch_bounds__(kStepBlock) void k_step(KParams k, Fused fu, Split 
     if (item >= 0) fused_block<FR>(k, fu, item, itile, L);
     return;
   }
-  // the other parts work in kBlock threads; the rest of the block has no work
-  if (threadIdx.x >= kBlock) return;
-  b -= ln.nfused;
+  // the other parts work in kBlock threads: two of their items per block
+  const int tid = threadIdx.x & (kBlock - 1);
+  b = 2 * (b - ln.nfused) + (threadIdx.x >= kBlock);
   if (b < ln.nrest) {
     int i, l;
-    if (rest_cell(sp, b * kBlock + threadIdx.x, &i, &l)) stress_cell(k, i, l, itile);
+    if (rest_cell(sp, b * kBlock + tid, &i, &l)) stress_cell(k, i, l, itile);
     return;
   }
   b -= ln.nrest;
   if (b < ln.nruns) {
     int tile;
     const int r = run_rect(sp, xcd_tile(b, ln.nruns), &tile);
-    stress_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
-                   threadIdx.x & 63, itile);
+    stress_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, tid >> 6,
+                   tid & 63, itile);
     return;
   }
   b -= ln.nruns;
-  sample_receivers(k, b * kBlock + threadIdx.x, ln.sample_step, true);
+  if (b < ln.nsample) sample_receivers(k, b * kBlock + tid, ln.sample_step, true);
 }
 
 // Two-step sweep of steps itile, itile+1 over the region: set A -> set C,
@@ -674,7 +676,7 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       // the samples of step it-1 ride along with the first launch of step it
       ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
       ln.sample_step = it - 1;
-      const int g1 = ln.nfused + ln.nrest + ln.nruns + ln.nsample;
+      const int g1 = ln.nfused + (ln.nrest + ln.nruns + ln.nsample + 1) / 2;
       if (g1) {
         auto launch = [&](auto kern) {
           hipLaunchKernelGGL(kern, dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu, h->ss, ln, it);
