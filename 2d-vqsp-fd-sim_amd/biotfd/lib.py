@@ -23,7 +23,8 @@ EXPORTS = (
     "bq_upload_bottom_nr", "bq_set_source", "bq_set_receivers", "bq_run", "bq_sync",
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells",
-    "bq_set_sweep", "bq_get_sweep", "bq_kernel_stats",
+    "bq_set_sweep", "bq_get_sweep", "bq_kernel_stats", "bq_set_tiles", "bq_get_tiles",
+    "bq_autotune",
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
     "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
     "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
@@ -31,6 +32,9 @@ EXPORTS = (
 )
 
 SCHED_GRID, SCHED_PERSISTENT = 0, 1
+# bq_set_tiles modes, and the candidates of bq_autotune in order
+TILES = {"auto": 0, "fused": 1, "cells": 2}
+TUNE_TILES = ("fused128", "fused64", "fused32", "fused16", "cells")
 # candidate c of bq_batch_autotune: tile height TUNE_ROWS[c // 2], schedule c % 2
 TUNE_ROWS = (64, 128, 256)
 
@@ -83,6 +87,9 @@ def load(path: str = LIBPATH):
         "bq_kernel_cells": ([p, p], C.c_int),
         "bq_set_sweep": ([p, C.c_int], C.c_int),
         "bq_get_sweep": ([p], C.c_int),
+        "bq_set_tiles": ([p, C.c_int], C.c_int),
+        "bq_get_tiles": ([p], C.c_int),
+        "bq_autotune": ([p, C.c_int, p], C.c_int),
         "bq_kernel_stats": ([p, p, p, p], C.c_int),
         "bq_build_bottom_nr": ([p, p, p, p], C.c_int),
         "bq_nonref": ([C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, p, p],
@@ -230,6 +237,24 @@ class Simulation:
     def sweep(self) -> int:
         """Steps per sweep the next run uses (bq_get_sweep)."""
         return int(self.lib.bq_get_sweep(self.h))
+
+    def set_tiles(self, mode: str):
+        """Fused tiles or not (bq_set_tiles): "auto" (the default), "fused" or
+        "cells"."""
+        self._chk(self.lib.bq_set_tiles(self.h, TILES[mode]))
+
+    def tiles(self) -> str:
+        """Partition the next run uses: "fused" or "cells" (bq_get_tiles)."""
+        return {1: "fused", 2: "cells"}[int(self.lib.bq_get_tiles(self.h))]
+
+    def autotune(self, trial_steps: int = 4) -> dict:
+        """Runs 5 * (trial_steps + 1) real steps, one set per partition, and
+        keeps the fastest (bq_autotune). Returns ms per step by candidate:
+        fused blocks of 128, 64, 32, 16 rows, then "cells"."""
+        ms = np.zeros(5, np.float64)
+        self._chk(self.lib.bq_autotune(self.h, int(trial_steps), _ptr(ms)))
+        self.step += 5 * (int(trial_steps) + 1)
+        return dict(zip(TUNE_TILES, map(float, ms)))
 
     def kernel_stats(self) -> dict:
         """Timing of the last timed run by launch kind (bq_kernel_stats)."""

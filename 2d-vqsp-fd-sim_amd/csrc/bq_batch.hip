@@ -390,22 +390,25 @@ int layout(bq_batch *b, int fr) {
   for (int s = 0; s < n; ++s) {
     const bq_handle *h = b->shots[s];
     ShotDev &d = b->host[s];
-    d.fu = h->fu;
+    // the partition with fused tiles, whatever the shot's own bq_run uses:
+    // the batch's launch holds enough tiles of all shots to fill the GPU
+    const bq::Plan &p = h->tiles;
+    d.fu = p.fu;
     bq::fused_rows(d.fu, fr);
-    d.ss = h->ss;
-    d.sv = h->sv;
+    d.ss = p.ss;
+    d.sv = p.sv;
     d.nfused = bq::fused_blocks(d.fu);
-    d.nrest = (h->ss.cum[kMaxCells] + kBlock - 1) / kBlock;
-    d.nruns = h->ss.qcum[kMaxRuns];
+    d.nrest = (p.ss.cum[kMaxCells] + kBlock - 1) / kBlock;
+    d.nruns = p.ss.qcum[kMaxRuns];
     d.nsample = (h->mr + kBlock - 1) / kBlock;
-    d.vrest = (h->sv.cum[kMaxCells] + kBlock - 1) / kBlock;
-    d.vruns = h->sv.qcum[kMaxRuns];
-    d.nedge = h->sv.fold ? 0 : (h->g.NX + kEdgeBlock - 1) / kEdgeBlock;
+    d.vrest = (p.sv.cum[kMaxCells] + kBlock - 1) / kBlock;
+    d.vruns = p.sv.qcum[kMaxRuns];
+    d.nedge = p.sv.fold ? 0 : (h->g.NX + kEdgeBlock - 1) / kEdgeBlock;
     d.lb_on = h->lb != nullptr;
     d.nlb = d.lb_on ? 2 * lb_tiles(lb_lz(h->g.MZ)) + lb_tiles(lb_lx(h->g.MX)) : 0;
     d.nlbx = d.lb_on ? (int)((lb_band_size(h->g.MX, h->g.MZ) + kBlock - 1) / kBlock) : 0;
     const int cnt[C_N] = {d.nfused, d.nrest + d.nruns + d.nsample,
-                          h->sv.etop + h->sv.ebot + d.vrest + d.vruns, d.nedge, d.nsample, d.nlb, d.nlbx,
+                          p.sv.etop + p.sv.ebot + d.vrest + d.vruns, d.nedge, d.nsample, d.nlb, d.nlbx,
                           d.fu.nfx * d.fu.nfz, d.fu.nb2, d.fu.nb4};
     for (int c = 0; c < C_N; ++c) cum[(size_t)c * (n + 1) + s + 1] = cum[(size_t)c * (n + 1) + s] + cnt[c];
   }

@@ -181,6 +181,37 @@ void dfree(bq_handle *h, void *p) {
   h->allocs.erase(it);
 }
 
+// Fused blocks of height 16 below which BQ_TILES_AUTO leaves a grid without
+// fused tiles. A fused block marches its rows one after another; with few of
+// them the launch is as long as one block's march and most CUs idle, while
+// the runs and the per-cell path spread the same cells over many short
+// blocks. Measured on MI355X: 1500 x 800 cells (250 blocks) and 2000 x 1000
+// (378) step faster without fused tiles, 2500 x 1200 (525) with them.
+constexpr int kTilesMin = 448;
+
+// Fused tile heights bq_create and bq_autotune choose from.
+constexpr int kTuneRows[4] = {128, 64, 32, 16};
+
+// The partition with fused tiles of fr rows per block.
+bq::Plan tiles_plan(const bq_handle *h, int fr) {
+  return bq::plan_cells(h->g, h->kp.OFF, kFW, fr, kRun, h->z0, h->z0);
+}
+
+// The partition bq_run uses: the one with fused tiles for BQ_TILES_FUSED, the
+// one without for BQ_TILES_CELLS, and for BQ_TILES_AUTO the one with tiles
+// where the grid holds at least kTilesMin blocks of height 16.
+void select_tiles(bq_handle *h) {
+  const bool fused = h->tile_mode == BQ_TILES_FUSED || (h->tile_mode == BQ_TILES_AUTO && h->tiles_auto);
+  const bq::Plan &p = fused ? h->tiles : h->cells;
+  h->fu = p.fu;
+  h->ss = p.ss;
+  h->sv = p.sv;
+  h->fr = h->fr_tiles;
+  h->fused_cells = p.fused_cells;
+  h->stress_cells = p.stress_cells;
+  h->velocity_cells = p.velocity_cells;
+}
+
 }  // namespace
 
 // Message of the last failed bq_create on this thread (there is no handle to
@@ -316,25 +347,29 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   // k_edges', row 0 of the velocities is never updated; without it every row
   // from 0 is a per-cell row
   const int z0 = k.SI ? 2 : 0;
-  // rows per fused block: the tallest of 128, 64, 32, 16 that still gives
-  // about two blocks per CU. A block marches its rows one after another, so
-  // on a grid with few tiles shorter blocks finish sooner; on a large grid
-  // taller ones recompute fewer halo rows.
-  bq::Plan plan{};
-  int fr = 16;
-  for (int c : {128, 64, 32, 16}) {
-    plan = bq::plan_cells(g, k.OFF, kFW, c, kRun, z0, z0);
-    fr = c;
-    const int nb = bq::fused_blocks(plan.fu);
-    if (nb == 0 || nb >= kFusedTarget) break;
+  // rows per fused block: the one of 128, 64, 32, 16 whose launch
+  // bq::fused_estimate finds shortest. A block marches its rows one after
+  // another, so on a grid with few tiles shorter blocks finish sooner; on a
+  // large grid taller ones recompute fewer halo rows.
+  h->z0 = z0;
+  bq::Plan plan = tiles_plan(h, kTuneRows[0]);
+  int fr = kTuneRows[0];
+  double best = bq::fused_estimate(plan.fu, fr);
+  for (int c = 1; c < 4; ++c) {
+    const double e = bq::fused_estimate(plan.fu, kTuneRows[c]);
+    if (e < best) {
+      best = e;
+      fr = kTuneRows[c];
+    }
   }
-  h->fr = fr;
-  h->fu = plan.fu;
-  h->ss = plan.ss;
-  h->sv = plan.sv;
-  h->fused_cells = plan.fused_cells;
-  h->stress_cells = plan.stress_cells;
-  h->velocity_cells = plan.velocity_cells;
+  bq::fused_rows(plan.fu, fr);
+  h->tiles = plan;
+  h->fr_tiles = fr;
+  bq::Fused f16 = plan.fu;
+  bq::fused_rows(f16, 16);
+  h->tiles_auto = bq::fused_blocks(f16) >= kTilesMin;
+  h->cells = bq::plan_cells(g, k.OFF, kFW, fr, kRun, z0, z0, true, false);
+  select_tiles(h);
   // two steps per sweep where the grid has a region for it: its partition and
   // the scratch set (cleared, like the others)
   const bq::Plan2 plan2 = bq::plan_cells2(g, k.OFF, kF2W, kF2R, kRun, z0, z0);
@@ -592,6 +627,72 @@ int bq_set_receivers(bq_handle *h, int mr, const int32_t *irecx, const int32_t *
     if (!ok(irecx[j], lrecx[j]) || !ok(irecz[j], lrecz[j]) || !ok(irecz[j], lrecx[j]))
       h->rec_sweep_ok = false;
   return 0;
+}
+
+int bq_set_tiles(bq_handle *h, int mode) {
+  if (!h) return -1;
+  if (mode != BQ_TILES_AUTO && mode != BQ_TILES_FUSED && mode != BQ_TILES_CELLS)
+    return fail(h, -2, "bq_set_tiles: mode must be BQ_TILES_AUTO, BQ_TILES_FUSED or BQ_TILES_CELLS");
+  h->tile_mode = mode;
+  select_tiles(h);
+  return 0;
+}
+
+int bq_get_tiles(bq_handle *h) {
+  if (!h) return -1;
+  return bq::fused_blocks(h->fu) ? BQ_TILES_FUSED : BQ_TILES_CELLS;
+}
+
+int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
+  if (!h) return -1;
+  if (trial_steps < 1) return fail(h, -2, "bq_autotune: trial_steps must be at least 1");
+  if (!h->model_ok || !h->pml_ok || !h->src_ok)
+    return fail(h, -6, "model, PML and source must be set before bq_autotune");
+  if (h->next_step - 1 + 5LL * (trial_steps + 1) > h->g.MT2)
+    return fail(h, -7, "bq_autotune: the trial steps run beyond MT2");
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto &f : ev)
+        if (f) (void)hipEventDestroy(f);
+      return fail(h, -100, "bq_autotune: hipEventCreate failed");
+    }
+  // the tiles apply to single steps, and the trials keep out of bq_kernel_times
+  const int sweep = h->sweep, timing = h->timing;
+  h->sweep = 1;
+  h->timing = 0;
+  double ms[5] = {0, 0, 0, 0, 0};
+  int rc = 0;
+  for (int c = 0; c < 5 && rc == 0; ++c) {
+    if (c < 4) {
+      h->tiles = tiles_plan(h, kTuneRows[c]);
+      h->fr_tiles = kTuneRows[c];
+    }
+    h->tile_mode = c < 4 ? BQ_TILES_FUSED : BQ_TILES_CELLS;
+    select_tiles(h);
+    // one step first, so each candidate's timed steps start warm
+    rc = bq_run(h, h->next_step, h->next_step);
+    if (rc == 0 && hipEventRecord(ev[0], h->stream) != hipSuccess) rc = fail(h, -100, "hipEventRecord");
+    if (rc == 0) rc = bq_run(h, h->next_step, h->next_step + trial_steps - 1);
+    float t = 0.f;
+    if (rc == 0 && (hipEventRecord(ev[1], h->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess))
+      rc = fail(h, -100, "bq_autotune: event timing failed");
+    ms[c] = (double)t / trial_steps;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  h->sweep = sweep;
+  h->timing = timing;
+  int best = 0;
+  for (int c = 1; c < 4; ++c)
+    if (ms[c] < ms[best]) best = c;
+  h->tiles = tiles_plan(h, kTuneRows[best]);
+  h->fr_tiles = kTuneRows[best];
+  h->tile_mode = rc == 0 && ms[4] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
+  select_tiles(h);
+  if (ms_per_step)
+    for (int c = 0; c < 5; ++c) ms_per_step[c] = ms[c];
+  return rc;
 }
 
 int bq_set_sweep(bq_handle *h, int steps) {

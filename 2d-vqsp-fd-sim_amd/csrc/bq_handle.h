@@ -35,6 +35,15 @@ struct bq_handle {
   std::vector<hipEvent_t> ev;
   double ktime[3] = {0, 0, 0};
   int timed_steps = 0;
+  // the two partitions of a step: with fused tiles of fr_tiles rows (the
+  // batches always take this one) and without them, every cell on the runs
+  // and the per-cell path; tile_mode (bq_set_tiles) picks the one bq_run uses
+  bq::Plan tiles{}, cells{};
+  int fr_tiles = 128;
+  int tile_mode = 0;
+  bool tiles_auto = true;  // BQ_TILES_AUTO takes the fused tiles
+  int z0 = 2;              // first row of both half-steps' cells (2 with stress imaging, else 0)
+  // the partition bq_run uses
   Split ss{}, sv{};
   Fused fu{};
   int fr = 128;  // rows per fused block of the single-grid launches

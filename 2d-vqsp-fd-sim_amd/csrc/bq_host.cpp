@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "bq_internal.h"
@@ -160,6 +162,33 @@ void fused_rows(Fused &fu, int fr) {
   fu.nb4 = (fu.n4 && rows > 0) ? fu.n4 * cdiv(cdiv(rows, fr / 4), 4) : 0;
 }
 
+double fused_estimate(const Fused &fu0, int fr) {
+  constexpr int kSlots = 512;
+  constexpr double kRowUs = 1.9, kRowPerBlockUs = 0.0039;
+  Fused fu = fu0;
+  fused_rows(fu, fr);
+  const int rows = fu.vz1 - fu.vz0;
+  // row steps of each block in dispatch order
+  std::vector<int> d;
+  d.reserve(fused_blocks(fu));
+  for (int bz = 0; bz < fu.nfz; ++bz) d.insert(d.end(), fu.nfx, std::min(fr, rows - bz * fr) + 4);
+  d.insert(d.end(), fu.nb2, fr / 2 + 4);
+  d.insert(d.end(), fu.nb4, fr / 4 + 4);
+  // every running block advances at the same rate: keep each block's end in
+  // row steps of progress, and the time per row step for the running count
+  std::priority_queue<double, std::vector<double>, std::greater<double>> ends;
+  double t = 0.0, prog = 0.0;
+  size_t next = 0;
+  while (next < d.size() || !ends.empty()) {
+    while (ends.size() < (size_t)kSlots && next < d.size()) ends.push(prog + d[next++]);
+    const double e = ends.top();
+    t += (e - prog) * (kRowUs + kRowPerBlockUs * (double)ends.size());
+    prog = e;
+    while (!ends.empty() && ends.top() <= prog) ends.pop();
+  }
+  return t;
+}
+
 // Interior columns start at the first column >= 5 whose address is 128-byte
 // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
 // Fused tiles cover columns [fx, vxe), vxe <= MX-3, and rows [5, MZ'-3): their
@@ -194,7 +223,8 @@ static bool edge_blocks(const GridDesc &g, int zv, Split &sv) {
   return true;
 }
 
-Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold) {
+Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold,
+                bool fused) {
   Plan p{};
   const int PX1 = g.PMLXH + 1, ZS1 = g.PMLZH + 1, ZV1 = g.PMLZH;  // exclusive ends
   int fx = 5;
@@ -204,7 +234,7 @@ Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int 
   fu.vx0 = fx;
   fu.vz0 = 5;
   fu.vz1 = g.MZ - 3;
-  const bool rows = fu.vz1 > fu.vz0 && ntx > 0;
+  const bool rows = fused && fu.vz1 > fu.vz0 && ntx > 0;
   const int width = rows && g.MX - 3 > fx ? g.MX - 3 - fx : 0;
   fu.nfx = width / fw;
   int rest = width - fw * fu.nfx;

@@ -85,6 +85,14 @@ struct Fused {
 // Sets nfz, nb2, nb4 of fu for block height fr.
 void fused_rows(Fused &fu, int fr);
 
+// Estimated duration (us) of a fused launch of block height fr: the blocks
+// are dispatched in class order (full, half, quarter) into 512 slots (two
+// blocks per CU), each marches its rows plus 4 halo rows one after another,
+// and a row takes 1.9 us + 3.9 ns per running block (latency-bound with few
+// blocks, bandwidth-bound with many; fitted on MI355X timings of grids from
+// 2500 x 1200 to 8000 x 4000 cells at heights 16 to 128).
+double fused_estimate(const Fused &fu, int fr);
+
 // Blocks of a fused launch: full, half and quarter tiles.
 inline int fused_blocks(const Fused &fu) { return fu.nfx * fu.nfz + fu.nb2 + fu.nb4; }
 
@@ -109,8 +117,10 @@ struct Plan {
 // (the rows outside these are the kernels' own edge rows). off = -PMLL; fw,
 // fr: fused tile width and height; nr: rows per wavefront run. fold: the
 // velocity half-step takes the edge rows too (Split::fold), where the grid
-// allows it.
-Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold = true);
+// allows it. fused: false leaves every cell to the runs and the per-cell
+// path (no fused tiles).
+Plan plan_cells(const GridDesc &g, int off, int fw, int fr, int nr, int zs, int zv, bool fold = true,
+                bool fused = true);
 
 // Two-step region: own cells [x0, x1) x [z0, z1) in nfx x nfz tiles of the
 // tile width and fr rows (the last row of tiles clipped); nfx = 0: none.

@@ -668,7 +668,6 @@ __device__ __forceinline__ void velocity_run(const KParams &k, int x0, int xl, i
 constexpr int kBlock = 256;  // 4 wavefronts
 constexpr int kRun = 4;      // rows per wavefront run
 constexpr int kFR = 128;     // rows per fused block (the tallest; bq_create picks per grid)
-constexpr int kFusedTarget = 448;  // fused blocks wanted per launch (about two per CU)
 
 __device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l) {
   if (g >= sp.cum[kMaxCells]) return false;

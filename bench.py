@@ -49,6 +49,7 @@ B_STEP_CELL = 184
 B_HALF_CELL = 88
 B_FUSED_MIN_CELL = 64 + 88
 TIMING_STRIDE = 8     # kernel durations from HIP events around every 8th step
+TUNE_TRIAL = 4        # --tiles tune: timed steps per bq_autotune candidate
 
 
 def make_workload(steps: int, mx=8000, mz=4000, per_cell=False):
@@ -396,6 +397,10 @@ def main():
     ap.add_argument("--sweep", type=int, choices=(1, 2), default=1,
                     help="roofline workload: steps per sweep (1, default: one step per launch "
                          "sequence; 2: the interior two steps per pass, bq_set_sweep)")
+    ap.add_argument("--tiles", choices=("auto", "fused", "cells", "tune"), default="auto",
+                    help="roofline workload: fused interior tiles or not (bq_set_tiles; auto: "
+                         "fused where the grid has enough of them; tune: bq_autotune over "
+                         "25 of the warm-up steps)")
     ap.add_argument("--materials", choices=("indexed", "cell"), default="indexed",
                     help="roofline workload: quantized material table (default) or one "
                          "material per cell (184 B per cell-step basis; no CPU baseline)")
@@ -429,8 +434,19 @@ def main():
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     sim = lib.Simulation(job)
     sim.set_sweep(args.sweep)
-    print(f"[bench] uploaded; {sim.sweep()} step(s) per sweep", file=sys.stderr, flush=True)
-    sim.run(args.warmup)
+    tune = None
+    warm = args.warmup
+    if args.tiles == "tune":
+        if warm < 5 * (TUNE_TRIAL + 1):
+            raise SystemExit(f"--tiles tune needs --warmup {5 * (TUNE_TRIAL + 1)} or more")
+        tune = {c: round(v, 5) for c, v in sim.autotune(TUNE_TRIAL).items()}
+        warm -= 5 * (TUNE_TRIAL + 1)
+        print(f"[bench] autotune ms/step: {tune}", file=sys.stderr, flush=True)
+    else:
+        sim.set_tiles(args.tiles)
+    tiles = sim.tiles()
+    print(f"[bench] uploaded; {sim.sweep()} step(s) per sweep, {tiles} tiles", file=sys.stderr, flush=True)
+    sim.run(warm)
     print("[bench] warmup done", file=sys.stderr, flush=True)
     # HIP events around the launches of every 8th step of the timed region
     sim.enable_timing(TIMING_STRIDE)
@@ -468,7 +484,8 @@ def main():
     achieved = dom_bytes / dom_s / 1e9
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tr_path) and not per_cell:
+    # the counted traffic of the 8000 x 4000 grid's launches with fused tiles
+    if os.path.exists(tr_path) and not per_cell and (args.mx, args.mz) == (8000, 4000) and tiles == "fused":
         with open(tr_path) as f:
             traffic = json.load(f).get(dom_name)
     if traffic is not None:
@@ -512,6 +529,8 @@ def main():
         # whole step, all launches: B_alg x NXc x NZc per step over ms_per_step
         "hbm_roofline_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "steps_per_sweep": sweep,
+        "tiles": tiles,
+        **({"autotune_ms_per_step": tune} if tune else {}),
         "kernel_ms": {n: round(t, 5) for n, t in ktimes.items()},
         "kernel_cells": kcells,
         "roofline": {"bound": "hbm", "kernel": dom_name,

@@ -1,6 +1,7 @@
-// Exhaustive check of the cell partition (bq::plan_cells) on a list of grids:
-// every cell of each half-step is updated by exactly one of the fused tiles,
-// the runs and the per-cell path, and every kernel's reads stay in the grid.
+// Exhaustive check of the cell partition (bq::plan_cells) on a list of grids,
+// with and without fused tiles: every cell of each half-step is updated by
+// exactly one of the fused tiles, the runs and the per-cell path, and every
+// kernel's reads stay in the grid.
 // Usage: plan_check MX MZ' TPML [MX MZ' TPML ...]; prints "ok <grids>".
 #include <cstdio>
 #include <cstdlib>
@@ -15,12 +16,13 @@ static int fail(const char *what, int mx, int mz, int t, int x, int z) {
   return 1;
 }
 
-static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv) {
+static int check(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv, bool fused) {
   GridDesc g{};
   g.MX = MX; g.MZ = MZ; g.T = T; g.PMLL = 1 - T; g.PMLXH = MX + T; g.PMLZH = MZ + T;
   g.NX = MX + 2 * T; g.NZ = MZ + T + 1;
   const int off = T - 1;
-  const Plan p = plan_cells(g, off, fw, fr, nr, zs, zv);
+  const Plan p = plan_cells(g, off, fw, fr, nr, zs, zv, true, fused);
+  if (!fused && (fused_blocks(p.fu) != 0 || p.fused_cells != 0)) return fail("tiles left", MX, MZ, T, 0, 0);
   const int NX = g.NX, NZ = g.NZ;
   std::vector<int> cs((size_t)NX * NZ, 0), cv((size_t)NX * NZ, 0);
   auto at = [&](std::vector<int> &v, int x, int z) -> int & { return v[(size_t)z * NX + (x - g.PMLL)]; };
@@ -220,7 +222,9 @@ int main(int argc, char **argv) {
   const int zs = std::atoi(argv[4]), zv = std::atoi(argv[5]);
   int n = 0;
   for (int a = 6; a + 2 < argc; a += 3) {
-    if (check(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), fw, fr, nr, zs, zv)) return 1;
+    for (bool fused : {true, false})
+      if (check(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), fw, fr, nr, zs, zv, fused))
+        return 1;
     for (int w2 : {256, 192})
       if (check2(std::atoi(argv[a]), std::atoi(argv[a + 1]), std::atoi(argv[a + 2]), w2, fr, nr, zs, zv))
         return 1;

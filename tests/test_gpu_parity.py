@@ -28,19 +28,27 @@ CASES = {**jobs.ALL, **jobs.GPU_EXTRA}
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_bitexact_vs_oracle(name):
+    """Each case twice, beside one oracle run: with the interior in fused tiles
+    and with every cell on the runs and the per-cell path (bq_set_tiles)."""
     job = CASES[name]()
     print(f"case {name}: grid {job.mx}x{job.mz} T={job.tpml} steps={job.mt2}", flush=True)
     ref = oracle.OracleSim(job, "f32")
-    sim = lib.Simulation(job)
-    np.testing.assert_array_equal(sim.sourtf, ref.sourtf)
+    sims = {}
+    for mode in ("fused", "cells"):
+        sims[mode] = lib.Simulation(job)
+        sims[mode].set_tiles(mode)
+    assert sims["cells"].tiles() == "cells"
+    np.testing.assert_array_equal(sims["fused"].sourtf, ref.sourtf)
     done = 0
     for chunk in (1, 9, job.mt2 - 10):
         ref.run(chunk)
-        sim.run(chunk)
         done += chunk
-        _compare(sim, ref, f"{name} step {done}")
-    np.testing.assert_array_equal(sim.traces(), ref.traces())
-    assert np.abs(sim.field("U")).max() > 0
+        for mode, sim in sims.items():
+            sim.run(chunk)
+            _compare(sim, ref, f"{name} ({mode} tiles) step {done}")
+    for sim in sims.values():
+        np.testing.assert_array_equal(sim.traces(), ref.traces())
+        assert np.abs(sim.field("U")).max() > 0
 
 
 def test_pml_coefficients_match_oracle():
@@ -123,6 +131,47 @@ def test_kernel_timing_leaves_results_unchanged(every, tpml):
         np.testing.assert_array_equal(a.field(f), b.field(f))
     a.close()
     b.close()
+
+
+def test_autotune_switches_partitions_bit_exact():
+    """bq_autotune runs real steps with fused blocks of 128, 64, 32 and 16
+    rows and without fused tiles, switching between them from step to step,
+    and keeps one: the run stays bit-identical to the oracle."""
+    job = jobs.wide_deep(steps=70)
+    print(f"autotune: grid {job.mx}x{job.mz}", flush=True)
+    ref = oracle.OracleSim(job, "f32")
+    sim = lib.Simulation(job)
+    ms = sim.autotune(2)
+    assert list(ms) == list(lib.TUNE_TILES)
+    assert all(v > 0 for v in ms.values()), ms
+    assert sim.step == 15
+    best = min(ms, key=ms.get)
+    assert sim.tiles() == ("cells" if best == "cells" else "fused"), (ms, sim.tiles())
+    ref.run(15)
+    _compare(sim, ref, "after the autotuner")
+    sim.run(job.mt2 - 15)
+    ref.run(job.mt2 - 15)
+    _compare(sim, ref, "end of run")
+    np.testing.assert_array_equal(sim.traces(), ref.traces())
+
+
+def test_tiles_modes_and_errors():
+    """BQ_TILES_AUTO leaves a small grid without fused tiles, BQ_TILES_FUSED and
+    BQ_TILES_CELLS force the partition; bad modes and autotuner calls that
+    would run past MT2 are rejected."""
+    job = jobs.wide_deep(steps=20)
+    sim = lib.Simulation(job)
+    assert sim.tiles() == "cells"
+    sim.set_tiles("fused")
+    assert sim.tiles() == "fused"
+    sim.set_tiles("cells")
+    assert sim.tiles() == "cells"
+    sim.set_tiles("auto")
+    assert sim.tiles() == "cells"
+    assert sim.lib.bq_set_tiles(sim.h, 3) == -2
+    assert sim.lib.bq_autotune(sim.h, 0, None) == -2
+    assert sim.lib.bq_autotune(sim.h, 4, None) == -7   # 25 steps > MT2 = 20
+    sim.close()
 
 
 def test_set_receivers_replaced_before_first_step_rejected_after():
