@@ -34,7 +34,7 @@ EXPORTS = (
 SCHED_GRID, SCHED_PERSISTENT = 0, 1
 # bq_set_tiles modes, and the candidates of bq_autotune in order
 TILES = {"auto": 0, "fused": 1, "cells": 2}
-TUNE_TILES = ("fused128", "fused64", "fused32", "fused16", "cells")
+TUNE_TILES = ("fused128", "fused64", "fused48", "fused32", "fused24", "fused16", "cells")
 # candidate c of bq_batch_autotune: tile height TUNE_ROWS[c // 2], schedule c % 2
 TUNE_ROWS = (64, 128, 256)
 
@@ -248,12 +248,12 @@ class Simulation:
         return {1: "fused", 2: "cells"}[int(self.lib.bq_get_tiles(self.h))]
 
     def autotune(self, trial_steps: int = 4) -> dict:
-        """Runs 5 * (trial_steps + 1) real steps, one set per partition, and
-        keeps the fastest (bq_autotune). Returns ms per step by candidate:
-        fused blocks of 128, 64, 32, 16 rows, then "cells"."""
-        ms = np.zeros(5, np.float64)
+        """Runs len(TUNE_TILES) * (trial_steps + 1) real steps, one set per
+        partition, and keeps the fastest (bq_autotune). Returns ms per step by
+        candidate: fused blocks of 128 .. 16 rows, then "cells"."""
+        ms = np.zeros(len(TUNE_TILES), np.float64)
         self._chk(self.lib.bq_autotune(self.h, int(trial_steps), _ptr(ms)))
-        self.step += 5 * (int(trial_steps) + 1)
+        self.step += len(TUNE_TILES) * (int(trial_steps) + 1)
         return dict(zip(TUNE_TILES, map(float, ms)))
 
     def kernel_stats(self) -> dict:

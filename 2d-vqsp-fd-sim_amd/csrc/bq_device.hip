@@ -189,8 +189,11 @@ void dfree(bq_handle *h, void *p) {
 // (378) step faster without fused tiles, 2500 x 1200 (525) with them.
 constexpr int kTilesMin = 448;
 
-// Fused tile heights bq_create and bq_autotune choose from.
-constexpr int kTuneRows[4] = {128, 64, 32, 16};
+// Fused tile heights bq_create and bq_autotune choose from (bq_autotune's
+// candidates 0..kNumRows-1; candidate kNumRows is the partition without
+// fused tiles).
+constexpr int kNumRows = 6;
+constexpr int kTuneRows[kNumRows] = {128, 64, 48, 32, 24, 16};
 
 // The partition with fused tiles of fr rows per block.
 bq::Plan tiles_plan(const bq_handle *h, int fr) {
@@ -355,7 +358,7 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   bq::Plan plan = tiles_plan(h, kTuneRows[0]);
   int fr = kTuneRows[0];
   double best = bq::fused_estimate(plan.fu, fr);
-  for (int c = 1; c < 4; ++c) {
+  for (int c = 1; c < kNumRows; ++c) {
     const double e = bq::fused_estimate(plan.fu, kTuneRows[c]);
     if (e < best) {
       best = e;
@@ -648,7 +651,7 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   if (trial_steps < 1) return fail(h, -2, "bq_autotune: trial_steps must be at least 1");
   if (!h->model_ok || !h->pml_ok || !h->src_ok)
     return fail(h, -6, "model, PML and source must be set before bq_autotune");
-  if (h->next_step - 1 + 5LL * (trial_steps + 1) > h->g.MT2)
+  if (h->next_step - 1 + (kNumRows + 1LL) * (trial_steps + 1) > h->g.MT2)
     return fail(h, -7, "bq_autotune: the trial steps run beyond MT2");
   hipEvent_t ev[2] = {nullptr, nullptr};
   for (auto &e : ev)
@@ -661,14 +664,14 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   const int sweep = h->sweep, timing = h->timing;
   h->sweep = 1;
   h->timing = 0;
-  double ms[5] = {0, 0, 0, 0, 0};
+  double ms[kNumRows + 1] = {};
   int rc = 0;
-  for (int c = 0; c < 5 && rc == 0; ++c) {
-    if (c < 4) {
+  for (int c = 0; c <= kNumRows && rc == 0; ++c) {
+    if (c < kNumRows) {
       h->tiles = tiles_plan(h, kTuneRows[c]);
       h->fr_tiles = kTuneRows[c];
     }
-    h->tile_mode = c < 4 ? BQ_TILES_FUSED : BQ_TILES_CELLS;
+    h->tile_mode = c < kNumRows ? BQ_TILES_FUSED : BQ_TILES_CELLS;
     select_tiles(h);
     // one step first, so each candidate's timed steps start warm
     rc = bq_run(h, h->next_step, h->next_step);
@@ -684,14 +687,14 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   h->sweep = sweep;
   h->timing = timing;
   int best = 0;
-  for (int c = 1; c < 4; ++c)
+  for (int c = 1; c < kNumRows; ++c)
     if (ms[c] < ms[best]) best = c;
   h->tiles = tiles_plan(h, kTuneRows[best]);
   h->fr_tiles = kTuneRows[best];
-  h->tile_mode = rc == 0 && ms[4] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
+  h->tile_mode = rc == 0 && ms[kNumRows] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
   select_tiles(h);
   if (ms_per_step)
-    for (int c = 0; c < 5; ++c) ms_per_step[c] = ms[c];
+    for (int c = 0; c <= kNumRows; ++c) ms_per_step[c] = ms[c];
   return rc;
 }
 
@@ -784,7 +787,9 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
         };
         switch (h->fr) {
           case 16: launch(k_step<kRun, 16>); break;
+          case 24: launch(k_step<kRun, 24>); break;
           case 32: launch(k_step<kRun, 32>); break;
+          case 48: launch(k_step<kRun, 48>); break;
           case 64: launch(k_step<kRun, 64>); break;
           default: launch(k_step<kRun, 128>); break;
         }

@@ -400,7 +400,7 @@ def main():
     ap.add_argument("--tiles", choices=("auto", "fused", "cells", "tune"), default="auto",
                     help="roofline workload: fused interior tiles or not (bq_set_tiles; auto: "
                          "fused where the grid has enough of them; tune: bq_autotune over "
-                         "25 of the warm-up steps)")
+                         "35 of the warm-up steps)")
     ap.add_argument("--materials", choices=("indexed", "cell"), default="indexed",
                     help="roofline workload: quantized material table (default) or one "
                          "material per cell (184 B per cell-step basis; no CPU baseline)")
@@ -437,10 +437,11 @@ def main():
     tune = None
     warm = args.warmup
     if args.tiles == "tune":
-        if warm < 5 * (TUNE_TRIAL + 1):
-            raise SystemExit(f"--tiles tune needs --warmup {5 * (TUNE_TRIAL + 1)} or more")
+        n_tune = len(lib.TUNE_TILES) * (TUNE_TRIAL + 1)
+        if warm < n_tune:
+            raise SystemExit(f"--tiles tune needs --warmup {n_tune} or more")
         tune = {c: round(v, 5) for c, v in sim.autotune(TUNE_TRIAL).items()}
-        warm -= 5 * (TUNE_TRIAL + 1)
+        warm -= n_tune
         print(f"[bench] autotune ms/step: {tune}", file=sys.stderr, flush=True)
     else:
         sim.set_tiles(args.tiles)

@@ -280,8 +280,8 @@ int bq_get_sweep(bq_handle *h);
  * least 448 fused blocks of 16 rows (about 2500 x 1200 cells and up): on a
  * smaller grid a launch lasts as long as one block's march down its rows
  * while most CUs idle. BQ_TILES_FUSED and BQ_TILES_CELLS force one or the
- * other. bq_create picks the fused blocks' height (128, 64, 32 or 16 rows)
- * from a model of the launch. The mode may be changed between runs. Batches
+ * other. bq_create picks the fused blocks' height (128, 64, 48, 32, 24 or 16
+ * rows) from a model of the launch. The mode may be changed between runs. Batches
  * (bq_batch_create) always use the fused tiles. bq_get_tiles returns
  * BQ_TILES_FUSED or BQ_TILES_CELLS for the partition bq_run uses
  * (BQ_TILES_CELLS where the grid has no fused tiles). */
@@ -289,13 +289,13 @@ enum { BQ_TILES_AUTO = 0, BQ_TILES_FUSED = 1, BQ_TILES_CELLS = 2 };
 int bq_set_tiles(bq_handle *h, int mode);
 int bq_get_tiles(bq_handle *h);
 /* Autotuner (no counterpart in the reference): runs 1 + trial_steps real
- * steps with each of five partitions - fused blocks of 128, 64, 32 and 16
- * rows, then no fused tiles (candidate c: height 128 >> c for c < 4) - times
- * the last trial_steps of each with HIP events and keeps the fastest
+ * steps with each of seven partitions - fused blocks of 128, 64, 48, 32, 24
+ * and 16 rows (candidates 0-5), then no fused tiles (candidate 6) - times the
+ * last trial_steps of each with HIP events and keeps the fastest
  * (bq_set_tiles(h, BQ_TILES_FUSED or BQ_TILES_CELLS) with the fastest
- * height). The 5 * (trial_steps + 1) steps advance the run like any others
+ * height). The 7 * (trial_steps + 1) steps advance the run like any others
  * (every candidate gives identical results); they are single steps whatever
- * bq_set_sweep allows. ms_per_step[5], if not NULL, receives each candidate's
+ * bq_set_sweep allows. ms_per_step[7], if not NULL, receives each candidate's
  * device time per step. */
 int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step);
 /* Timing of the last timed bq_run by launch kind, ms per launch. With two

@@ -134,9 +134,9 @@ def test_kernel_timing_leaves_results_unchanged(every, tpml):
 
 
 def test_autotune_switches_partitions_bit_exact():
-    """bq_autotune runs real steps with fused blocks of 128, 64, 32 and 16
-    rows and without fused tiles, switching between them from step to step,
-    and keeps one: the run stays bit-identical to the oracle."""
+    """bq_autotune runs real steps with fused blocks of 128, 64, 48, 32, 24 and
+    16 rows and without fused tiles, switching between them from step to
+    step, and keeps one: the run stays bit-identical to the oracle."""
     job = jobs.wide_deep(steps=70)
     print(f"autotune: grid {job.mx}x{job.mz}", flush=True)
     ref = oracle.OracleSim(job, "f32")
@@ -144,13 +144,13 @@ def test_autotune_switches_partitions_bit_exact():
     ms = sim.autotune(2)
     assert list(ms) == list(lib.TUNE_TILES)
     assert all(v > 0 for v in ms.values()), ms
-    assert sim.step == 15
+    assert sim.step == 21
     best = min(ms, key=ms.get)
     assert sim.tiles() == ("cells" if best == "cells" else "fused"), (ms, sim.tiles())
-    ref.run(15)
+    ref.run(21)
     _compare(sim, ref, "after the autotuner")
-    sim.run(job.mt2 - 15)
-    ref.run(job.mt2 - 15)
+    sim.run(job.mt2 - 21)
+    ref.run(job.mt2 - 21)
     _compare(sim, ref, "end of run")
     np.testing.assert_array_equal(sim.traces(), ref.traces())
 
@@ -170,7 +170,7 @@ def test_tiles_modes_and_errors():
     assert sim.tiles() == "cells"
     assert sim.lib.bq_set_tiles(sim.h, 3) == -2
     assert sim.lib.bq_autotune(sim.h, 0, None) == -2
-    assert sim.lib.bq_autotune(sim.h, 4, None) == -7   # 25 steps > MT2 = 20
+    assert sim.lib.bq_autotune(sim.h, 2, None) == -7   # 21 steps > MT2 = 20
     sim.close()
 
 
