@@ -181,20 +181,6 @@ void dfree(bq_handle *h, void *p) {
   h->allocs.erase(it);
 }
 
-// Fused blocks of height 16 below which BQ_TILES_AUTO leaves a grid without
-// fused tiles. A fused block marches its rows one after another; with few of
-// them the launch is as long as one block's march and most CUs idle, while
-// the runs and the per-cell path spread the same cells over many short
-// blocks. Measured on MI355X: 1500 x 800 cells (250 blocks) and 2000 x 1000
-// (378) step faster without fused tiles, 2500 x 1200 (525) with them.
-constexpr int kTilesMin = 448;
-
-// Fused tile heights bq_create and bq_autotune choose from (bq_autotune's
-// candidates 0..kNumRows-1; candidate kNumRows is the partition without
-// fused tiles).
-constexpr int kNumRows = 6;
-constexpr int kTuneRows[kNumRows] = {128, 64, 48, 32, 24, 16};
-
 // The partition with fused tiles of fr rows per block.
 bq::Plan tiles_plan(const bq_handle *h, int fr) {
   return bq::plan_cells(h->g, h->kp.OFF, kFW, fr, kRun, h->z0, h->z0);
@@ -202,7 +188,7 @@ bq::Plan tiles_plan(const bq_handle *h, int fr) {
 
 // The partition bq_run uses: the one with fused tiles for BQ_TILES_FUSED, the
 // one without for BQ_TILES_CELLS, and for BQ_TILES_AUTO the one with tiles
-// where the grid holds at least kTilesMin blocks of height 16.
+// where bq::fused_worth finds enough of them.
 void select_tiles(bq_handle *h) {
   const bool fused = h->tile_mode == BQ_TILES_FUSED || (h->tile_mode == BQ_TILES_AUTO && h->tiles_auto);
   const bq::Plan &p = fused ? h->tiles : h->cells;
@@ -350,27 +336,17 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   // k_edges', row 0 of the velocities is never updated; without it every row
   // from 0 is a per-cell row
   const int z0 = k.SI ? 2 : 0;
-  // rows per fused block: the one of 128, 64, 32, 16 whose launch
-  // bq::fused_estimate finds shortest. A block marches its rows one after
-  // another, so on a grid with few tiles shorter blocks finish sooner; on a
-  // large grid taller ones recompute fewer halo rows.
+  // rows per fused block: the height whose launch bq::fused_estimate finds
+  // shortest. A block marches its rows one after another, so on a grid with
+  // few tiles shorter blocks finish sooner; on a large grid taller ones
+  // recompute fewer halo rows.
   h->z0 = z0;
-  bq::Plan plan = tiles_plan(h, kTuneRows[0]);
-  int fr = kTuneRows[0];
-  double best = bq::fused_estimate(plan.fu, fr);
-  for (int c = 1; c < kNumRows; ++c) {
-    const double e = bq::fused_estimate(plan.fu, kTuneRows[c]);
-    if (e < best) {
-      best = e;
-      fr = kTuneRows[c];
-    }
-  }
+  bq::Plan plan = tiles_plan(h, bq::kTileRows[0]);
+  const int fr = bq::fused_pick(plan.fu);
   bq::fused_rows(plan.fu, fr);
   h->tiles = plan;
   h->fr_tiles = fr;
-  bq::Fused f16 = plan.fu;
-  bq::fused_rows(f16, 16);
-  h->tiles_auto = bq::fused_blocks(f16) >= kTilesMin;
+  h->tiles_auto = bq::fused_worth(plan.fu);
   h->cells = bq::plan_cells(g, k.OFF, kFW, fr, kRun, z0, z0, true, false);
   select_tiles(h);
   // two steps per sweep where the grid has a region for it: its partition and
@@ -651,7 +627,7 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   if (trial_steps < 1) return fail(h, -2, "bq_autotune: trial_steps must be at least 1");
   if (!h->model_ok || !h->pml_ok || !h->src_ok)
     return fail(h, -6, "model, PML and source must be set before bq_autotune");
-  if (h->next_step - 1 + (kNumRows + 1LL) * (trial_steps + 1) > h->g.MT2)
+  if (h->next_step - 1 + (bq::kNumRows + 1LL) * (trial_steps + 1) > h->g.MT2)
     return fail(h, -7, "bq_autotune: the trial steps run beyond MT2");
   hipEvent_t ev[2] = {nullptr, nullptr};
   for (auto &e : ev)
@@ -664,14 +640,14 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   const int sweep = h->sweep, timing = h->timing;
   h->sweep = 1;
   h->timing = 0;
-  double ms[kNumRows + 1] = {};
+  double ms[bq::kNumRows + 1] = {};
   int rc = 0;
-  for (int c = 0; c <= kNumRows && rc == 0; ++c) {
-    if (c < kNumRows) {
-      h->tiles = tiles_plan(h, kTuneRows[c]);
-      h->fr_tiles = kTuneRows[c];
+  for (int c = 0; c <= bq::kNumRows && rc == 0; ++c) {
+    if (c < bq::kNumRows) {
+      h->tiles = tiles_plan(h, bq::kTileRows[c]);
+      h->fr_tiles = bq::kTileRows[c];
     }
-    h->tile_mode = c < kNumRows ? BQ_TILES_FUSED : BQ_TILES_CELLS;
+    h->tile_mode = c < bq::kNumRows ? BQ_TILES_FUSED : BQ_TILES_CELLS;
     select_tiles(h);
     // one step first, so each candidate's timed steps start warm
     rc = bq_run(h, h->next_step, h->next_step);
@@ -687,14 +663,14 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   h->sweep = sweep;
   h->timing = timing;
   int best = 0;
-  for (int c = 1; c < kNumRows; ++c)
+  for (int c = 1; c < bq::kNumRows; ++c)
     if (ms[c] < ms[best]) best = c;
-  h->tiles = tiles_plan(h, kTuneRows[best]);
-  h->fr_tiles = kTuneRows[best];
-  h->tile_mode = rc == 0 && ms[kNumRows] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
+  h->tiles = tiles_plan(h, bq::kTileRows[best]);
+  h->fr_tiles = bq::kTileRows[best];
+  h->tile_mode = rc == 0 && ms[bq::kNumRows] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
   select_tiles(h);
   if (ms_per_step)
-    for (int c = 0; c <= kNumRows; ++c) ms_per_step[c] = ms[c];
+    for (int c = 0; c <= bq::kNumRows; ++c) ms_per_step[c] = ms[c];
   return rc;
 }
 

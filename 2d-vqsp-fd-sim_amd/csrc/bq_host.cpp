@@ -189,6 +189,25 @@ double fused_estimate(const Fused &fu0, int fr) {
   return t;
 }
 
+int fused_pick(const Fused &fu) {
+  int fr = kTileRows[0];
+  double best = fused_estimate(fu, fr);
+  for (int c = 1; c < kNumRows; ++c) {
+    const double e = fused_estimate(fu, kTileRows[c]);
+    if (e < best) {
+      best = e;
+      fr = kTileRows[c];
+    }
+  }
+  return fr;
+}
+
+bool fused_worth(const Fused &fu) {
+  Fused f = fu;
+  fused_rows(f, 16);
+  return fused_blocks(f) >= kTilesMin;
+}
+
 // Interior columns start at the first column >= 5 whose address is 128-byte
 // aligned, so a wavefront's 64 values of a row are exactly two cache lines.
 // Fused tiles cover columns [fx, vxe), vxe <= MX-3, and rows [5, MZ'-3): their

@@ -93,6 +93,25 @@ void fused_rows(Fused &fu, int fr);
 // 2500 x 1200 to 8000 x 4000 cells at heights 16 to 128).
 double fused_estimate(const Fused &fu, int fr);
 
+// Fused block heights the planner and the autotuner choose from.
+constexpr int kNumRows = 6;
+constexpr int kTileRows[kNumRows] = {128, 64, 48, 32, 24, 16};
+
+// The height of kTileRows whose launch fused_estimate finds shortest (the
+// taller one on a tie).
+int fused_pick(const Fused &fu);
+
+// Fused blocks of height 16 below which the automatic choice leaves a grid
+// without fused tiles. A fused block marches its rows one after another; with
+// few of them the launch is as long as one block's march and most CUs idle,
+// while the runs and the per-cell path spread the same cells over many short
+// blocks. Measured on MI355X: 1500 x 800 cells (250 blocks) and 2000 x 1000
+// (378) step faster without fused tiles, 2500 x 1200 (525) with them.
+constexpr int kTilesMin = 448;
+
+// Whether the automatic choice takes the fused tiles of fu's columns.
+bool fused_worth(const Fused &fu);
+
 // Blocks of a fused launch: full, half and quarter tiles.
 inline int fused_blocks(const Fused &fu) { return fu.nfx * fu.nfz + fu.nb2 + fu.nb4; }
 

@@ -5,6 +5,7 @@
 // Usage: plan_check MX MZ' TPML [MX MZ' TPML ...]; prints "ok <grids>".
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "bq_internal.h"
@@ -213,7 +214,34 @@ static int check2(int MX, int MZ, int T, int fw, int fr, int nr, int zs, int zv)
   return 0;
 }
 
+// plan_check pick MX MZ' T ...: the fused block height bq_create takes and
+// whether the automatic choice keeps the fused tiles, one line per grid.
+static int pick(int argc, char **argv) {
+  for (int a = 2; a + 2 < argc; a += 3) {
+    GridDesc g{};
+    g.MX = std::atoi(argv[a]); g.MZ = std::atoi(argv[a + 1]); g.T = std::atoi(argv[a + 2]);
+    g.PMLL = 1 - g.T; g.PMLXH = g.MX + g.T; g.PMLZH = g.MZ + g.T;
+    g.NX = g.MX + 2 * g.T; g.NZ = g.MZ + g.T + 1;
+    const Plan p = plan_cells(g, g.T - 1, 512, kTileRows[0], 4, 2, 2);
+    std::printf("%d %d\n", fused_pick(p.fu), fused_worth(p.fu) ? 1 : 0);
+  }
+  return 0;
+}
+
+// plan_check est NFX ROWS FR: fused_estimate of nfx columns of full tiles
+// over rows rows (no half or quarter tiles), in us.
+static int est(char **argv) {
+  Fused f{};
+  f.nfx = std::atoi(argv[2]);
+  f.vz0 = 0;
+  f.vz1 = std::atoi(argv[3]);
+  std::printf("%.6f\n", fused_estimate(f, std::atoi(argv[4])));
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc >= 2 && std::string(argv[1]) == "pick") return pick(argc, argv);
+  if (argc == 5 && std::string(argv[1]) == "est") return est(argv);
   if (argc < 9 || (argc - 6) % 3 != 0) {
     std::printf("usage: plan_check FW FR NR ZS ZV MX MZ T [MX MZ T ...]\n");
     return 2;

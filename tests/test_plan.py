@@ -38,3 +38,38 @@ def test_partition_covers_each_cell_once(checker, fw, fr, nr, zs, zv):
     r = subprocess.run(args, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip() == f"ok {len(grids())}"
+
+
+# grid (physical MX x MZ, TPML 20, stress imaging: MZ' = MZ + 2) -> the fused
+# block height bq_create takes, or 0 where the automatic choice leaves the grid
+# without fused tiles. Timed on MI355X (bench.py --mx X --mz Z with each height
+# forced): the model's choice is the fastest or within 1 % of it on each grid.
+PICKS = {(400, 400): 0, (1500, 800): 0, (2000, 1000): 0, (2500, 1200): 16, (3000, 1500): 24,
+         (4000, 2000): 32, (5000, 2500): 48, (6000, 3000): 24, (8000, 4000): 128}
+
+
+def test_tile_height_and_partition_choice(checker):
+    args = [checker, "pick"]
+    for mx, mz in PICKS:
+        args += [str(mx), str(mz + 2), "20"]
+    r = subprocess.run(args, capture_output=True, text=True, check=True)
+    got = [tuple(map(int, line.split())) for line in r.stdout.split("\n") if line]
+    for (grid, want), (fr, worth) in zip(PICKS.items(), got):
+        assert (fr if worth else 0) == want, (grid, fr, worth)
+
+
+def _est(checker, nfx, rows, fr):
+    r = subprocess.run([checker, "est", str(nfx), str(rows), str(fr)], capture_output=True, text=True,
+                       check=True)
+    return float(r.stdout)
+
+
+def test_launch_estimate_closed_forms(checker):
+    """n equal blocks of fr + 4 row steps: one wave of n <= 512 blocks takes
+    (fr + 4)(1.9 + 0.0039 n) us; 1024 take two full waves; a clipped last row
+    of blocks is shorter."""
+    for n in (1, 7, 512):
+        assert _est(checker, n, 64, 64) == pytest.approx(68 * (1.9 + 0.0039 * n), rel=1e-9)
+    assert _est(checker, 512, 128, 64) == pytest.approx(2 * 68 * (1.9 + 0.0039 * 512), rel=1e-9)
+    # 2 blocks: 64 + 4 and 16 + 4 row steps; both run for 20, then one for 48
+    assert _est(checker, 1, 80, 64) == pytest.approx(20 * (1.9 + 0.0078) + 48 * (1.9 + 0.0039), rel=1e-9)
