@@ -346,7 +346,7 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   bq::fused_rows(plan.fu, fr);
   h->tiles = plan;
   h->fr_tiles = fr;
-  h->tiles_auto = bq::fused_worth(plan.fu);
+  h->tiles_auto = bq::fused_worth(g, plan.fu);
   h->cells = bq::plan_cells(g, k.OFF, kFW, fr, kRun, z0, z0, true, false);
   select_tiles(h);
   // two steps per sweep where the grid has a region for it: its partition and
@@ -757,7 +757,11 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       ln.nsample = (it > itile_begin && k.MR > 0) ? nsb : 0;
       ln.sample_step = it - 1;
       const int g1 = ln.nfused + (ln.nrest + ln.nruns + ln.nsample + 1) / 2;
-      if (g1) {
+      if (ln.nfused == 0) {
+        // no fused tiles: one item per 256-thread block, twice the blocks
+        const int gs = ln.nrest + ln.nruns + ln.nsample;
+        if (gs) hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, k, h->ss, ln, it);
+      } else if (g1) {
         auto launch = [&](auto kern) {
           hipLaunchKernelGGL(kern, dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu, h->ss, ln, it);
         };

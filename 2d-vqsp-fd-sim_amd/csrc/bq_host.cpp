@@ -202,10 +202,10 @@ int fused_pick(const Fused &fu) {
   return fr;
 }
 
-bool fused_worth(const Fused &fu) {
+bool fused_worth(const GridDesc &g, const Fused &fu) {
   Fused f = fu;
   fused_rows(f, 16);
-  return fused_blocks(f) >= kTilesMin;
+  return fused_blocks(f) > 0 && (long long)g.NX * g.NZ > kCellsMax;
 }
 
 // Interior columns start at the first column >= 5 whose address is 128-byte

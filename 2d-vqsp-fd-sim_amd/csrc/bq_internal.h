@@ -101,16 +101,19 @@ constexpr int kTileRows[kNumRows] = {128, 64, 48, 32, 24, 16};
 // taller one on a tie).
 int fused_pick(const Fused &fu);
 
-// Fused blocks of height 16 below which the automatic choice leaves a grid
-// without fused tiles. A fused block marches its rows one after another; with
-// few of them the launch is as long as one block's march and most CUs idle,
-// while the runs and the per-cell path spread the same cells over many short
-// blocks. Measured on MI355X: 1500 x 800 cells (250 blocks) and 2000 x 1000
-// (378) step faster without fused tiles, 2500 x 1200 (525) with them.
-constexpr int kTilesMin = 448;
+// Grid cells (NXc x NZc) up to which the automatic choice leaves a grid
+// without fused tiles. The runs and the per-cell path read and write both
+// field sets once per half-step; while those (64 B per cell) stay within
+// about the 256 MB last-level cache they outrun the fused tiles, whose
+// blocks march their rows one after another and leave CUs idle on a small
+// grid. Measured on MI355X: 4.33 M cells (3040 x 1423) step faster without
+// fused tiles (39.9 against 39.3 Gcell/s), 4.63 M (3040 x 1523) with them
+// (40.4 against 38.6).
+constexpr long long kCellsMax = 4500000;
 
-// Whether the automatic choice takes the fused tiles of fu's columns.
-bool fused_worth(const Fused &fu);
+// Whether the automatic choice takes the fused tiles (of fu's columns, on
+// grid g): where g holds more than kCellsMax cells and fu any fused block.
+bool fused_worth(const GridDesc &g, const Fused &fu);
 
 // Blocks of a fused launch: full, half and quarter tiles.
 inline int fused_blocks(const Fused &fu) { return fu.nfx * fu.nfz + fu.nb2 + fu.nb4; }

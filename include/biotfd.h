@@ -276,10 +276,10 @@ int bq_get_sweep(bq_handle *h);
  * cell alike, time_loop.F90:65-206). Each step's interior is either marched in
  * fused tiles (both half-steps per pass, k_step) or left to the runs and the
  * per-cell path of the two half-steps; results are bit-identical either way.
- * BQ_TILES_AUTO (the default) takes the fused tiles where the grid holds at
- * least 448 fused blocks of 16 rows (about 2500 x 1200 cells and up): on a
- * smaller grid a launch lasts as long as one block's march down its rows
- * while most CUs idle. BQ_TILES_FUSED and BQ_TILES_CELLS force one or the
+ * BQ_TILES_AUTO (the default) takes the fused tiles on grids of more than
+ * 4.5 M cells (NXc x NZc, about 3000 x 1500 and up): on a smaller grid a
+ * fused launch lasts as long as one block's march down its rows while CUs
+ * idle, and both field sets still fit about the last-level cache. BQ_TILES_FUSED and BQ_TILES_CELLS force one or the
  * other. bq_create picks the fused blocks' height (128, 64, 48, 32, 24 or 16
  * rows) from a model of the launch. The mode may be changed between runs. Batches
  * (bq_batch_create) always use the fused tiles. bq_get_tiles returns

@@ -223,7 +223,7 @@ static int pick(int argc, char **argv) {
     g.PMLL = 1 - g.T; g.PMLXH = g.MX + g.T; g.PMLZH = g.MZ + g.T;
     g.NX = g.MX + 2 * g.T; g.NZ = g.MZ + g.T + 1;
     const Plan p = plan_cells(g, g.T - 1, 512, kTileRows[0], 4, 2, 2);
-    std::printf("%d %d\n", fused_pick(p.fu), fused_worth(p.fu) ? 1 : 0);
+    std::printf("%d %d\n", fused_pick(p.fu), fused_worth(g, p.fu) ? 1 : 0);
   }
   return 0;
 }

@@ -44,8 +44,8 @@ def test_partition_covers_each_cell_once(checker, fw, fr, nr, zs, zv):
 # block height bq_create takes, or 0 where the automatic choice leaves the grid
 # without fused tiles. Timed on MI355X (bench.py --mx X --mz Z with each height
 # forced): the model's choice is the fastest or within 1 % of it on each grid.
-PICKS = {(400, 400): 0, (1500, 800): 0, (2000, 1000): 0, (2500, 1200): 16, (3000, 1500): 24,
-         (4000, 2000): 32, (5000, 2500): 48, (6000, 3000): 24, (8000, 4000): 128}
+PICKS = {(400, 400): 0, (1500, 800): 0, (2000, 1000): 0, (2500, 1200): 0, (3000, 1400): 0,
+         (3000, 1500): 24, (4000, 2000): 32, (5000, 2500): 48, (6000, 3000): 24, (8000, 4000): 128}
 
 
 def test_tile_height_and_partition_choice(checker):
