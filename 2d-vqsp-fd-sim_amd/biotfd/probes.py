@@ -12,7 +12,7 @@ from .lib import BiotError
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libbiotfd_probes.so")
-EXPORTS = ("bq_probe_copy", "bq_probe_march")
+EXPORTS = ("bq_probe_copy", "bq_probe_march", "bq_probe_bands")
 
 _lib = None
 
@@ -29,6 +29,8 @@ def load(path: str = LIBPATH):
     lib.bq_probe_copy.restype = C.c_int
     lib.bq_probe_march.argtypes = [C.c_int, C.c_int, C.c_int, p, p]
     lib.bq_probe_march.restype = C.c_int
+    lib.bq_probe_bands.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, p]
+    lib.bq_probe_bands.restype = C.c_int
     _lib = lib
     return lib
 
@@ -51,3 +53,14 @@ def probe_march(nx: int, nz: int, iters: int = 20):
     if rc:
         raise BiotError(f"bq_probe_march failed ({rc})")
     return float(out[0]), float(out[1])
+
+
+def probe_bands(nx: int, nz: int, bands: int, iters: int = 20) -> float:
+    """GB/s of the fused tiles' per-cell traffic (nine arrays read, eight
+    written) without the march, with the blocks in flight over `bands` row
+    bands (1: row-major)."""
+    out = np.zeros(1, dtype=np.float64)
+    rc = load().bq_probe_bands(int(nx), int(nz), int(bands), int(iters), out.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise BiotError(f"bq_probe_bands failed ({rc})")
+    return float(out[0])

@@ -50,6 +50,20 @@ struct MarchArrays {
   float *out[8];
 };
 
+// Band-order probe: block b takes segment b % nbx of row band (b / nbx) %
+// bands, row b / (nbx * bands) of that band (h rows per band), and reads nine
+// arrays and writes eight at its 512 cells.
+__global__ __launch_bounds__(kFW) void k_bands_probe(MarchArrays A, int pitch, int nbx, int bands, int h) {
+  const int b = blockIdx.x;
+  const int seg = b % nbx, band = (b / nbx) % bands, r = b / (nbx * bands);
+  const size_t i = (size_t)(band * h + r) * pitch + (size_t)seg * kFW + threadIdx.x;
+  float v = 0.0f;
+#pragma unroll
+  for (int f = 0; f < 9; ++f) v += A.in[f][i];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) A.out[f][i] = v;
+}
+
 __global__ __launch_bounds__(kFW) void k_march_probe(MarchArrays A, int pitch, int nbx) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % nbx, bz = tile / nbx;
@@ -131,6 +145,44 @@ int bq_probe_march(int nx, int nz, int iters, double *gbps, double *bytes) {
     const double moved = cols * nbz * (kFR + 4) * 9.0 * 4.0 + cols * nbz * kFR * 8.0 * 4.0;
     *gbps = moved * iters / (ms * 1e-3) / 1e9;
     if (bytes) *bytes = moved;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (hipGetLastError() != hipSuccess) rc = -100;
+  }
+  for (auto p : buf)
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int bq_probe_bands(int nx, int nz, int bands, int iters, double *gbps) {
+  if (nx < kFW || bands < 1 || bands > nz || iters < 1 || !gbps) return -1;
+  const int pitch = (nx + 63) / 64 * 64;
+  const int nbx = nx / kFW, h = nz / bands;
+  // rows touched: band * h + r < bands * h <= nz; columns < nbx * kFW <= nx <= pitch
+  const size_t n = (size_t)pitch * nz;
+  std::vector<float *> buf(17, nullptr);
+  int rc = 0;
+  for (auto &p : buf) {
+    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) { p = nullptr; rc = -101; break; }
+    (void)hipMemset(p, 0, n * sizeof(float));
+  }
+  if (rc == 0) {
+    MarchArrays A{};
+    for (int f = 0; f < 9; ++f) A.in[f] = buf[f];
+    for (int f = 0; f < 8; ++f) A.out[f] = buf[9 + f];
+    const int blocks = nbx * bands * h;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_bands_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx, bands, h);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < iters; ++r)
+      hipLaunchKernelGGL(k_bands_probe, dim3(blocks), dim3(kFW), 0, 0, A, pitch, nbx, bands, h);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *gbps = (double)blocks * kFW * 17.0 * 4.0 * iters / (ms * 1e-3) / 1e9;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (hipGetLastError() != hipSuccess) rc = -100;

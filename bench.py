@@ -405,8 +405,10 @@ def main():
                     help="roofline workload: quantized material table (default) or one "
                          "material per cell (184 B per cell-step basis; no CPU baseline)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="also time a plain 1 GiB device copy (k_copy_probe) and the fused "
-                         "tiles' access pattern without arithmetic (k_march_probe)")
+                    help="also time a plain 1 GiB device copy (k_copy_probe), the fused "
+                         "tiles' access pattern without arithmetic (k_march_probe) and its "
+                         "per-cell traffic without the march in 1 and in the march's number "
+                         "of row bands (k_bands_probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -492,11 +494,15 @@ def main():
     if traffic is not None:
         traffic = int(traffic)
 
-    copy_gbs = march_gbs = None
+    copy_gbs = march_gbs = bands_gbs = None
     if args.copy_probe:
         from biotfd import probes
         copy_gbs = probes.probe_copy(1 << 28, 20)
         march_gbs, _ = probes.probe_march(nx, nz, 20)
+        # the march's per-cell traffic without the march, in row-major order and
+        # over as many row bands as the march probe's rows of blocks
+        march_bands = max(1, (nz - 8) // 132)
+        bands_gbs = {str(nb): round(probes.probe_bands(nx, nz, nb, 20), 1) for nb in (1, march_bands)}
 
     cpu = cpu_all = None
     if not args.no_cpu_baseline and not per_cell:
@@ -550,6 +556,7 @@ def main():
     if copy_gbs is not None:
         out["copy_probe_gbs"] = round(copy_gbs, 1)
         out["march_probe_gbs"] = round(march_gbs, 1)
+        out["bands_probe_gbs"] = bands_gbs
     print(json.dumps(out), flush=True)
 
 

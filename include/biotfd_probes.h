@@ -26,6 +26,15 @@ int bq_probe_copy(long long nfloats, int iters, double *gbps);
  * nz >= 140. */
 int bq_probe_march(int nx, int nz, int iters, double *gbps, double *bytes);
 
+/* The same nine-read, eight-write traffic per cell without the march (kernel
+ * k_bands_probe, one cell per thread): each 512-thread block takes one
+ * 512-column segment of one row of [nz][nx] arrays, and the blocks are
+ * ordered so that those in flight cover `bands` row bands at once (bands = 1:
+ * row-major order; the fused tiles of a grid with R rows of blocks march in R
+ * bands). Reports read+write GB/s over columns [0, 512 * (nx / 512)) and the
+ * first bands * (nz / bands) rows. Needs nx >= 512, 1 <= bands <= nz. */
+int bq_probe_bands(int nx, int nz, int bands, int iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

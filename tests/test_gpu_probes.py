@@ -20,3 +20,12 @@ def test_march_probe_reports_bandwidth_and_exact_bytes():
 def test_march_probe_rejects_small_grids():
     with pytest.raises(lib.BiotError):
         probes.probe_march(100, 300, 1)
+
+
+def test_bands_probe_reports_bandwidth_and_rejects_bad_bands():
+    assert probes.probe_bands(1100, 300, 1, 2) > 0
+    assert probes.probe_bands(1100, 300, 7, 2) > 0   # 42 rows per band, 294 rows
+    with pytest.raises(lib.BiotError):
+        probes.probe_bands(1100, 300, 0, 1)
+    with pytest.raises(lib.BiotError):
+        probes.probe_bands(100, 300, 1, 1)
