@@ -636,8 +636,11 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
         if (f) (void)hipEventDestroy(f);
       return fail(h, -100, "bq_autotune: hipEventCreate failed");
     }
-  // the tiles apply to single steps, and the trials keep out of bq_kernel_times
+  // the tiles apply to single steps, and the trials keep out of bq_kernel_times;
+  // a failed trial leaves the partition as it was
   const int sweep = h->sweep, timing = h->timing;
+  const bq::Plan tiles0 = h->tiles;
+  const int fr0 = h->fr_tiles, mode0 = h->tile_mode;
   h->sweep = 1;
   h->timing = 0;
   double ms[bq::kNumRows + 1] = {};
@@ -662,12 +665,19 @@ int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   for (auto &e : ev) (void)hipEventDestroy(e);
   h->sweep = sweep;
   h->timing = timing;
+  if (rc) {
+    h->tiles = tiles0;
+    h->fr_tiles = fr0;
+    h->tile_mode = mode0;
+    select_tiles(h);
+    return rc;
+  }
   int best = 0;
   for (int c = 1; c < bq::kNumRows; ++c)
     if (ms[c] < ms[best]) best = c;
   h->tiles = tiles_plan(h, bq::kTileRows[best]);
   h->fr_tiles = bq::kTileRows[best];
-  h->tile_mode = rc == 0 && ms[bq::kNumRows] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
+  h->tile_mode = ms[bq::kNumRows] < ms[best] ? BQ_TILES_CELLS : BQ_TILES_FUSED;
   select_tiles(h);
   if (ms_per_step)
     for (int c = 0; c <= bq::kNumRows; ++c) ms_per_step[c] = ms[c];

@@ -296,7 +296,8 @@ int bq_get_tiles(bq_handle *h);
  * height). The 7 * (trial_steps + 1) steps advance the run like any others
  * (every candidate gives identical results); they are single steps whatever
  * bq_set_sweep allows. ms_per_step[7], if not NULL, receives each candidate's
- * device time per step. */
+ * device time per step. If a trial fails, the partition stays as it was and
+ * the steps already run stay run (bq_run's next step follows them). */
 int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step);
 /* Timing of the last timed bq_run by launch kind, ms per launch. With two
  * steps per sweep (*steps_per_sweep = 2): ms[0] the two-step sweep of the
