@@ -45,7 +45,7 @@ namespace {
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_velocity(KParams k, Split sp, int nrest, int nruns) {
   __shared__ float lds[kEdgeLds];
-  velocity_block(k, sp, blockIdx.x, nrest, nruns, lds);
+  velocity_block<NR>(k, sp, blockIdx.x, nrest, nruns, lds);
 }
 
 // First launch of a step: fused interior blocks (long-running, first), then
@@ -347,7 +347,9 @@ int bq_create(const bq_desc *d, bq_handle **out) {
   h->tiles = plan;
   h->fr_tiles = fr;
   h->tiles_auto = bq::fused_worth(g, plan.fu);
-  h->cells = bq::plan_cells(g, k.OFF, kFW, fr, kRun, z0, z0, true, false);
+  // runs of 2 rows per wavefront without fused tiles: these grids are small,
+  // and twice the run blocks finish sooner (4 rows: 3-15 % slower there)
+  h->cells = bq::plan_cells(g, k.OFF, kFW, fr, kRunCells, z0, z0, true, false);
   select_tiles(h);
   // two steps per sweep where the grid has a region for it: its partition and
   // the scratch set (cleared, like the others)
@@ -770,7 +772,12 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
       if (ln.nfused == 0) {
         // no fused tiles: one item per 256-thread block, twice the blocks
         const int gs = ln.nrest + ln.nruns + ln.nsample;
-        if (gs) hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, k, h->ss, ln, it);
+        if (gs) {
+          if (h->ss.nr == kRunCells)
+            hipLaunchKernelGGL(k_rest_stress<kRunCells>, dim3(gs), blk, 0, h->stream, k, h->ss, ln, it);
+          else
+            hipLaunchKernelGGL(k_rest_stress<kRun>, dim3(gs), blk, 0, h->stream, k, h->ss, ln, it);
+        }
       } else if (g1) {
         auto launch = [&](auto kern) {
           hipLaunchKernelGGL(kern, dim3(g1), dim3(kStepBlock), 0, h->stream, k, h->fu, h->ss, ln, it);
@@ -785,8 +792,12 @@ int bq_run(bq_handle *h, int itile_begin, int itile_end) {
         }
       }
       if (e) (void)hipEventRecord(e[1], h->stream);
-      if (vgrid)
-        hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest, vruns);
+      if (vgrid) {
+        if (h->sv.nr == kRunCells)
+          hipLaunchKernelGGL(k_velocity<kRunCells>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest, vruns);
+        else
+          hipLaunchKernelGGL(k_velocity<kRun>, dim3(vgrid), blk, 0, h->stream, k, h->sv, vrest, vruns);
+      }
       if (e) (void)hipEventRecord(e[2], h->stream);
       if (!h->sv.fold) {
         hipLaunchKernelGGL(k_edges, ge, dim3(kEdgeBlock), 0, h->stream, k);

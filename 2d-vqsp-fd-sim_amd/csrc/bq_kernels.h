@@ -667,6 +667,7 @@ __device__ __forceinline__ void velocity_run(const KParams &k, int x0, int xl, i
 
 constexpr int kBlock = 256;  // 4 wavefronts
 constexpr int kRun = 4;      // rows per wavefront run
+constexpr int kRunCells = 2; // rows per wavefront run of the partition without fused tiles
 constexpr int kFR = 128;     // rows per fused block (the tallest; bq_create picks per grid)
 
 __device__ __forceinline__ bool rest_cell(const Split &sp, int g, int *i, int *l) {
@@ -1271,6 +1272,7 @@ __device__ __forceinline__ void bottom_block(const KParams &k, int bx, float *ld
 
 // Block b of a velocity launch over split sp: the edge blocks (fold), then
 // nrest per-cell blocks, then nruns run blocks.
+template <int NR = kRun>
 __device__ __forceinline__ void velocity_block(const KParams &k, const Split &sp, int b, int nrest,
                                                int nruns, float *lds) {
   if (b < sp.etop) {
@@ -1290,7 +1292,7 @@ __device__ __forceinline__ void velocity_block(const KParams &k, const Split &sp
   }
   int tile;
   const int r = run_rect(sp, xcd_tile(b - nrest, nruns), &tile);
-  velocity_run<kRun>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
+  velocity_run<NR>(k, sp.qx0[r], sp.qxl[r], sp.qntx[r], sp.qz0[r], sp.qz1[r], tile, threadIdx.x >> 6,
                      threadIdx.x & 63);
 }
 

@@ -29,7 +29,8 @@ def grids():
     return g
 
 
-@pytest.mark.parametrize("fw,fr,nr", [(512, 128, 4), (512, 48, 4), (512, 24, 4), (256, 128, 4), (256, 32, 8)])
+@pytest.mark.parametrize("fw,fr,nr", [(512, 128, 4), (512, 48, 4), (512, 24, 4), (512, 16, 2), (256, 128, 4),
+                                      (256, 32, 8)])
 @pytest.mark.parametrize("zs,zv", [(2, 2), (0, 0)])
 def test_partition_covers_each_cell_once(checker, fw, fr, nr, zs, zv):
     args = [checker, str(fw), str(fr), str(nr), str(zs), str(zv)]
