@@ -116,3 +116,22 @@ def test_receiver_inside_the_region_keeps_single_steps():
     sim.set_sweep(2)
     assert sim.sweep() == 2
     sim.close()
+
+
+def test_autotune_runs_single_steps_and_keeps_the_sweep_setting():
+    """bq_autotune's trials are single steps whatever bq_set_sweep allows; the
+    two-step setting holds again after it, and the run stays bit-identical to
+    one that never left single steps."""
+    job = jobs.wide_deep(steps=50)
+    one, two = lib.Simulation(job), lib.Simulation(job)
+    two.set_sweep(2)
+    two.autotune(1)                 # 7 candidates x 2 steps
+    assert two.sweep() == 2
+    one.run(14)
+    _compare(two, one, "after the autotuner")
+    one.run(job.mt2 - 14)
+    two.run(job.mt2 - 14)
+    _compare(two, one, "end of run")
+    np.testing.assert_array_equal(two.traces(), one.traces())
+    one.close()
+    two.close()
