@@ -243,7 +243,9 @@ int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
 /* Average device time per step of each launch of the last bq_run, ms, from
  * HIP events on the handle's stream: out[0] k_step (both half-steps of the
  * interior cells, the stress half-step of all other cells and the previous
- * step's receiver samples), out[1] k_velocity (velocity half-step of the other
+ * step's receiver samples; on a partition without fused tiles, bq_set_tiles,
+ * k_rest_stress: the stress half-step of every cell and the samples), out[1]
+ * k_velocity (velocity half-step of the other
  * cells, with rows 1 and PMLZH where its edge blocks take them), out[2]
  * k_edges (rows 1 and PMLZH; 0 where k_velocity takes them: every grid with
  * TPML <= 124 and at least 3 rows between the surface and bottom edge blocks,
@@ -254,8 +256,8 @@ int bq_grid_shape(bq_handle *h, int32_t *nx, int32_t *nz);
 int bq_enable_timing(bq_handle *h, int every);
 int bq_kernel_times(bq_handle *h, double out[3]);
 /* Cells updated per step: out[0] by both half-steps inside k_step (fused
- * blocks), out[1] by the stress half-step of k_step outside them, out[2] by
- * k_velocity. */
+ * blocks; 0 without fused tiles), out[1] by the stress half-step of k_step
+ * outside them (of k_rest_stress without fused tiles), out[2] by k_velocity. */
 int bq_kernel_cells(bq_handle *h, long long out[3]);
 
 /* Two steps per sweep (no counterpart in the reference, whose loop advances
