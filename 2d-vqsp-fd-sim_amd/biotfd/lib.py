@@ -217,8 +217,9 @@ class Simulation:
 
     def kernel_times(self) -> dict:
         """ms per step of each launch: k_step (fused interior + boundary stress +
-        previous step's samples), k_velocity (boundary velocity, with the edge
-        rows where its edge blocks take them), k_edges (0 then)."""
+        previous step's samples; k_rest_stress without fused tiles), k_velocity
+        (boundary velocity, with the edge rows where its edge blocks take them),
+        k_edges (0 then)."""
         out = np.zeros(3, dtype=np.float64)
         self._chk(self.lib.bq_kernel_times(self.h, _ptr(out)))
         return dict(zip(("k_step", "k_velocity", "k_edges"), map(float, out)))
