@@ -61,3 +61,30 @@ def test_dropin_outputs_byte_identical(name, dropin, tmp_path):
         for la, lb in zip(a, b):
             ca, cb = la.split(), lb.split()
             assert [ca[k] for k in keep] == [cb[k] for k in keep], f
+
+
+@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(os.path.join(BUILD, "fd2d_gpu"))),
+                    reason="reference build or drop-in binary not built")
+def test_stop_file_present_at_start(tmp_path):
+    """A STOP file present before the loop (time_loop.F90:79-86): the reference
+    and the drop-in both stop before the first step, write the step number
+    into STOP and leave the same receiver and snapshot files. The drop-in
+    polls once per chunk of steps, so a STOP file that appears during a run
+    takes effect at the next chunk, not the next step."""
+    job = jobs.tiny_layered(steps=60)
+    job.key_tlv = job.key_tlq = job.key_tlp = job.key_snv = True
+    job.ipas2 = 25
+    outs = {}
+    for tag, exe in (("ref", REF), ("gpu", os.path.join(BUILD, "fd2d_gpu"))):
+        d = tmp_path / tag
+        formats.write_job(job, str(d))
+        (d / "STOP").write_text("")
+        print(f"running {tag} with STOP present", flush=True)
+        subprocess.run([exe], cwd=str(d), check=True, stdout=subprocess.DEVNULL, timeout=300)
+        outs[tag] = d
+    stop = {t: (d / "STOP").read_text().split() for t, d in outs.items()}
+    assert stop["ref"] == stop["gpu"] == ["1"], stop
+    files = {t: sorted(f for f in os.listdir(d) if f.startswith(("REC_", "SNAP."))) for t, d in outs.items()}
+    assert files["ref"] == files["gpu"], files
+    for f in files["ref"]:
+        assert filecmp.cmp(outs["ref"] / f, outs["gpu"] / f, shallow=False), f
