@@ -7,7 +7,9 @@
 * The layered half-space (4000 x 2000) and the roofline model (8000 x 4000,
   with quantized materials, with one material per cell, JMNUM = 32.0 M, and
   with two steps per sweep) on their full grids for a bounded number of
-  steps, bit for bit against the
+  steps, and the layered model on grids that take other partitions (fused
+  blocks of 24 and 48 rows; no fused tiles at 4.3 M cells), bit for bit
+  against the
   binary32 oracle on the host: from random initial fields, so that every cell
   of every row class - side and bottom PML strips, their memories, the
   surface rows, the bottom row and its history, every fused tile - is active
@@ -91,9 +93,16 @@ def _random_start(job, sim, ref, seed):
             ref.set_field(f + "P", a)
 
 
-def _cfg1_job(steps):
-    return models.layered_job(name="LAYER", mx=4000, mz=2000, h=2.0, steps=steps, n_layers=10,
+def _cfg1_job(steps, mx=4000, mz=2000):
+    return models.layered_job(name="LAYER", mx=mx, mz=mz, h=2.0, steps=steps, n_layers=10,
                               seed=1, tpml=20, f0=25.0, n_rec=1000)
+
+
+# grids whose default partition differs from the two workloads': fused blocks
+# of 24 and 48 rows (with half tiles of 12 / 24 rows and quarter tiles of 6 /
+# 12), and no fused tiles with 2-row runs just under the 4.5 M-cell limit
+OTHER_GRIDS = {"layered 6000x3000": (6000, 3000, "fused"), "layered 5000x2500": (5000, 2500, "fused"),
+               "layered 3000x1400": (3000, 1400, "cells")}
 
 
 def _cfg2_job(steps, per_cell=False):
@@ -103,18 +112,25 @@ def _cfg2_job(steps, per_cell=False):
 
 @pytest.mark.parametrize("which,steps,start", [
     ("layered 4000x2000", 24, "random"), ("layered 4000x2000", 300, "rest"),
+    ("layered 6000x3000", 6, "random"), ("layered 5000x2500", 6, "random"),
+    ("layered 3000x1400", 24, "random"),
     ("roofline 8000x4000", 12, "random"), ("roofline 8000x4000", 120, "rest"),
     ("roofline 8000x4000 one material per cell", 6, "random"),
     ("roofline 8000x4000 two steps per sweep", 13, "random"),
 ])
 def test_full_grid_bitexact_vs_oracle(which, steps, start):
     t0 = time.perf_counter()
-    job = (_cfg1_job(steps) if which.startswith("layered")
-           else _cfg2_job(steps, per_cell="per cell" in which))
+    if which in OTHER_GRIDS:
+        job = _cfg1_job(steps, *OTHER_GRIDS[which][:2])
+    else:
+        job = (_cfg1_job(steps) if which.startswith("layered")
+               else _cfg2_job(steps, per_cell="per cell" in which))
     nz, nx = hostsetup.grid_shape(job)
     print(f"{which} ({start}): {nx}x{nz} cells, JMNUM={job.table.shape[0]}, {steps} steps, "
           f"{len(job.receivers)} receivers", flush=True)
     sim = lib.Simulation(job)
+    if which in OTHER_GRIDS:
+        assert sim.tiles() == OTHER_GRIDS[which][2]
     if "two steps" in which:
         sim.set_sweep(2)
         assert sim.sweep() == 2
