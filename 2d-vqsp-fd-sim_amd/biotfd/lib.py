@@ -24,7 +24,7 @@ EXPORTS = (
     "bq_get_traces", "bq_get_field", "bq_set_field", "bq_source_from_stf", "bq_last_error",
     "bq_grid_shape", "bq_enable_timing", "bq_kernel_times", "bq_kernel_cells",
     "bq_set_sweep", "bq_get_sweep", "bq_kernel_stats", "bq_set_tiles", "bq_get_tiles",
-    "bq_autotune",
+    "bq_autotune", "bq_get_tile_rows",
     "bq_build_bottom_nr", "bq_nonref", "bq_write_seismograms", "bq_format_e13_6",
     "bq_batch_create", "bq_batch_destroy", "bq_batch_run", "bq_batch_sync", "bq_batch_last_run_ms",
     "bq_batch_autotune", "bq_batch_set_schedule", "bq_batch_get_schedule", "bq_batch_last_error",
@@ -90,6 +90,7 @@ def load(path: str = LIBPATH):
         "bq_set_tiles": ([p, C.c_int], C.c_int),
         "bq_get_tiles": ([p], C.c_int),
         "bq_autotune": ([p, C.c_int, p], C.c_int),
+        "bq_get_tile_rows": ([p], C.c_int),
         "bq_kernel_stats": ([p, p, p, p], C.c_int),
         "bq_build_bottom_nr": ([p, p, p, p], C.c_int),
         "bq_nonref": ([C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, p, p],
@@ -248,6 +249,10 @@ class Simulation:
         """Partition the next run uses: "fused" or "cells" (bq_get_tiles)."""
         return {1: "fused", 2: "cells"}[int(self.lib.bq_get_tiles(self.h))]
 
+    def tile_rows(self) -> int:
+        """Rows per fused block of the partition with fused tiles (bq_get_tile_rows)."""
+        return int(self.lib.bq_get_tile_rows(self.h))
+
     def autotune(self, trial_steps: int = 4) -> dict:
         """Runs len(TUNE_TILES) * (trial_steps + 1) real steps, one set per
         partition, and keeps the fastest (bq_autotune). Returns ms per step by
@@ -356,6 +361,10 @@ class Batch:
         out = np.zeros(1, np.float64)
         self._chk(self.lib.bq_batch_last_run_ms(self.b, _ptr(out)))
         return float(out[0])
+
+    def tile_rows(self) -> int:
+        """Rows per fused block of the partition with fused tiles (bq_get_tile_rows)."""
+        return int(self.lib.bq_get_tile_rows(self.h))
 
     def autotune(self, trial_steps: int = 4) -> dict:
         """Times the six candidates on the batch's own steps (6 * trial_steps

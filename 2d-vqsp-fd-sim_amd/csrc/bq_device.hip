@@ -624,6 +624,11 @@ int bq_get_tiles(bq_handle *h) {
   return bq::fused_blocks(h->fu) ? BQ_TILES_FUSED : BQ_TILES_CELLS;
 }
 
+int bq_get_tile_rows(bq_handle *h) {
+  if (!h) return -1;
+  return h->fr_tiles;
+}
+
 int bq_autotune(bq_handle *h, int trial_steps, double *ms_per_step) {
   if (!h) return -1;
   if (trial_steps < 1) return fail(h, -2, "bq_autotune: trial_steps must be at least 1");

@@ -542,6 +542,7 @@ def main():
         "hbm_roofline_frac_step": round(gcell * 1e9 * b_step / (HBM_PEAK_GBS * 1e9), 4),
         "steps_per_sweep": sweep,
         "tiles": tiles,
+        "tile_rows": sim.tile_rows() if tiles == "fused" else None,
         **({"autotune_ms_per_step": tune} if tune else {}),
         "kernel_ms": {n: round(t, 5) for n, t in ktimes.items()},
         "kernel_cells": kcells,

@@ -290,6 +290,10 @@ int bq_get_sweep(bq_handle *h);
 enum { BQ_TILES_AUTO = 0, BQ_TILES_FUSED = 1, BQ_TILES_CELLS = 2 };
 int bq_set_tiles(bq_handle *h, int mode);
 int bq_get_tiles(bq_handle *h);
+/* Rows per fused block of the partition with fused tiles (128, 64, 48, 32, 24
+ * or 16: bq_create's choice, or bq_autotune's), whether or not bq_run uses
+ * that partition. */
+int bq_get_tile_rows(bq_handle *h);
 /* Autotuner (no counterpart in the reference): runs 1 + trial_steps real
  * steps with each of seven partitions - fused blocks of 128, 64, 48, 32, 24
  * and 16 rows (candidates 0-5), then no fused tiles (candidate 6) - times the

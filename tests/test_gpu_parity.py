@@ -147,6 +147,8 @@ def test_autotune_switches_partitions_bit_exact():
     assert sim.step == 21
     best = min(ms, key=ms.get)
     assert sim.tiles() == ("cells" if best == "cells" else "fused"), (ms, sim.tiles())
+    fused = {k: v for k, v in ms.items() if k != "cells"}
+    assert sim.tile_rows() == int(min(fused, key=fused.get)[5:]), (ms, sim.tile_rows())
     ref.run(21)
     _compare(sim, ref, "after the autotuner")
     sim.run(job.mt2 - 21)
@@ -162,6 +164,7 @@ def test_tiles_modes_and_errors():
     job = jobs.wide_deep(steps=20)
     sim = lib.Simulation(job)
     assert sim.tiles() == "cells"
+    assert sim.tile_rows() in (128, 64, 48, 32, 24, 16)
     sim.set_tiles("fused")
     assert sim.tiles() == "fused"
     sim.set_tiles("cells")
