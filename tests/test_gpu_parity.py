@@ -133,12 +133,25 @@ def test_kernel_timing_leaves_results_unchanged(every, tpml):
     b.close()
 
 
-def test_autotune_switches_partitions_bit_exact():
+def _second_wide():
+    job = jobs.wide_deep(steps=70)
+    job.variant = 4
+    return job
+
+
+AUTOTUNE_CASES = {"wide": lambda: jobs.wide_deep(steps=70),
+                  "wide explicit surface": lambda: jobs.wide_deep(steps=70, stress_imaging=False),
+                  "wide second-order build": _second_wide,
+                  "narrow, masked quarter column": lambda: jobs.narrow_tiles(steps=70, mx=1000)}
+
+
+@pytest.mark.parametrize("case", list(AUTOTUNE_CASES))
+def test_autotune_switches_partitions_bit_exact(case):
     """bq_autotune runs real steps with fused blocks of 128, 64, 48, 32, 24 and
     16 rows and without fused tiles, switching between them from step to
     step, and keeps one: the run stays bit-identical to the oracle."""
-    job = jobs.wide_deep(steps=70)
-    print(f"autotune: grid {job.mx}x{job.mz}", flush=True)
+    job = AUTOTUNE_CASES[case]()
+    print(f"autotune {case}: grid {job.mx}x{job.mz}", flush=True)
     ref = oracle.OracleSim(job, "f32")
     sim = lib.Simulation(job)
     ms = sim.autotune(2)
