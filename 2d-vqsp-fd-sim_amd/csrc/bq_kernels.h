@@ -64,12 +64,16 @@ struct KParams {
 #define ZH1(t, j, ix) (k.z1[t][(size_t)(j) * k.NX + (ix)])
 #define ZH2(t, j, ix) (k.z2[t][(size_t)(j) * k.NX + (ix)])
 
-// A store of a value no kernel reads again in this launch, with the streaming
-// (non-temporal) hint: the new field rows the fused tiles write (8 per cell,
-// about 1 GB per step on an 8040 x 4023 grid) then do not displace the
-// material tables and halo rows from L2. Measured on that grid: k_step 493 us
-// against 519 us with plain stores.
+// Loads and stores with the streaming (non-temporal) hint, for the field rows
+// the fused tiles read in their own columns and the new rows they write (16
+// values per cell, about 2 GB per step on an 8040 x 4023 grid): they then do
+// not displace from L2 the material tables and the halo columns that
+// neighbouring tiles read at the same time. Measured on that grid, k_step:
+// 519 us with plain loads and stores, 493 us with streaming stores, 463-466 us
+// with streaming stores and own-column loads; the hint on the halo-column and
+// material-index loads as well made it 478 us.
 __device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
 
 // Material index of cell c (row-major with the field pitch), 0-based.
 __device__ __forceinline__ int jm_at(const KParams &k, size_t c) {
@@ -885,9 +889,11 @@ __device__ __forceinline__ void fused_march(const KParams &k, int c, int ws, int
   int fjm = 0, fhjm = 0;
   auto fetch = [&](int s) {
     const int r = s * P;
-    pu = U[r + P + gx]; pw = Wv[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
+    pu = stream_load(&U[r + P + gx]); pw = stream_load(&Wv[r + 2 * P + gx]);
+    pz = stream_load(&QZ[r + 2 * P + gx]); pq = stream_load(&QX[r + gx]);
     if (hl) { fhu = U[r + P + hgx]; fhw = Wv[r + 2 * P + hgx]; fhz = QZ[r + 2 * P + hgx]; fhq = QX[r + hgx]; }
-    ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
+    ftxx = stream_load(&k.TXX0[r + gx]); ftzz = stream_load(&k.TZZ0[r + gx]);
+    ftxz = stream_load(&k.TXZ0[r + gx]); fpr = stream_load(&k.PR0[r + gx]);
     fjm = jm_at(k, r + gx);
     if (hs) {
       fhtxx = k.TXX0[r + hgs]; fhtzz = k.TZZ0[r + hgs]; fhtxz = k.TXZ0[r + hgs];
