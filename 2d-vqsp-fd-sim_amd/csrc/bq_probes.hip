@@ -38,10 +38,15 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
   }
 }
 
+// Loads and stores with the streaming (non-temporal) hint, as the fused step
+// tiles read and write their own columns' field rows.
+__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 // Row-streaming probe: blocks of kFW columns x kFR rows march down nine input
 // arrays and write eight output arrays two rows behind, with no stencil work -
 // the access pattern of the fused step tiles without their arithmetic, one
-// float per lane per array and row. Columns [0, nbx*kFW), rows
+// float per lane per array and row, with the tiles' streaming hint. Columns [0, nbx*kFW), rows
 // [4, 4 + nbz*kFR + 4) of [nz][pitch] arrays. Bytes per launch are exact:
 // nbx*kFW*(nbz*(kFR+4))*9*4 read (the 4 halo rows of each block read again),
 // nbx*kFW*nbz*kFR*8*4 written.
@@ -52,16 +57,16 @@ struct MarchArrays {
 
 // Band-order probe: block b takes segment b % nbx of row band (b / nbx) %
 // bands, row b / (nbx * bands) of that band (h rows per band), and reads nine
-// arrays and writes eight at its 512 cells.
+// arrays and writes eight at its 512 cells, with the streaming hint.
 __global__ __launch_bounds__(kFW) void k_bands_probe(MarchArrays A, int pitch, int nbx, int bands, int h) {
   const int b = blockIdx.x;
   const int seg = b % nbx, band = (b / nbx) % bands, r = b / (nbx * bands);
   const size_t i = (size_t)(band * h + r) * pitch + (size_t)seg * kFW + threadIdx.x;
   float v = 0.0f;
 #pragma unroll
-  for (int f = 0; f < 9; ++f) v += A.in[f][i];
+  for (int f = 0; f < 9; ++f) v += stream_load(&A.in[f][i]);
 #pragma unroll
-  for (int f = 0; f < 8; ++f) A.out[f][i] = v;
+  for (int f = 0; f < 8; ++f) stream_store(&A.out[f][i], v);
 }
 
 __global__ __launch_bounds__(kFW) void k_march_probe(MarchArrays A, int pitch, int nbx) {
@@ -73,12 +78,12 @@ __global__ __launch_bounds__(kFW) void k_march_probe(MarchArrays A, int pitch, i
     const size_t r = (size_t)s * pitch + c;
     float v = 0.0f;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) v += A.in[f][r];
+    for (int f = 0; f < 9; ++f) v += stream_load(&A.in[f][r]);
     const int l = s - 2;
     if (l >= a && l < a + kFR) {
       const size_t rl = (size_t)l * pitch + c;
 #pragma unroll
-      for (int f = 0; f < 8; ++f) A.out[f][rl] = v;
+      for (int f = 0; f < 8; ++f) stream_store(&A.out[f][rl], v);
     }
   }
 }

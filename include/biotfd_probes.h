@@ -19,7 +19,8 @@ extern "C" {
 int bq_probe_copy(long long nfloats, int iters, double *gbps);
 
 /* The access pattern of the fused step tiles without their arithmetic
- * (kernel k_march_probe, one float per lane): blocks of 512 columns x 128
+ * (kernel k_march_probe, one float per lane, loads and stores with the
+ * streaming hint as the tiles' own columns use it): blocks of 512 columns x 128
  * rows march down nine [nz][nx] arrays (reading 4 halo rows more per block)
  * and write eight, one row at a time. Reports read+write GB/s and, if bytes
  * is not NULL, the exact bytes one launch reads and writes. Needs nx >= 512,
@@ -27,7 +28,7 @@ int bq_probe_copy(long long nfloats, int iters, double *gbps);
 int bq_probe_march(int nx, int nz, int iters, double *gbps, double *bytes);
 
 /* The same nine-read, eight-write traffic per cell without the march (kernel
- * k_bands_probe, one cell per thread): each 512-thread block takes one
+ * k_bands_probe, one cell per thread, streaming hint): each 512-thread block takes one
  * 512-column segment of one row of [nz][nx] arrays, and the blocks are
  * ordered so that those in flight cover `bands` row bands at once (bands = 1:
  * row-major order; the fused tiles of a grid with R rows of blocks march in R
