@@ -164,7 +164,7 @@ void fused_rows(Fused &fu, int fr) {
 
 double fused_estimate(const Fused &fu0, int fr) {
   constexpr int kSlots = 512;
-  constexpr double kRowUs = 1.9, kRowPerBlockUs = 0.0039;
+  constexpr double kRowUs = 1.9, kRowPerBlockUs = 0.0035;
   Fused fu = fu0;
   fused_rows(fu, fr);
   const int rows = fu.vz1 - fu.vz0;

@@ -88,9 +88,9 @@ void fused_rows(Fused &fu, int fr);
 // Estimated duration (us) of a fused launch of block height fr: the blocks
 // are dispatched in class order (full, half, quarter) into 512 slots (two
 // blocks per CU), each marches its rows plus 4 halo rows one after another,
-// and a row takes 1.9 us + 3.9 ns per running block (latency-bound with few
+// and a row takes 1.9 us + 3.5 ns per running block (latency-bound with few
 // blocks, bandwidth-bound with many; fitted on MI355X timings of grids from
-// 2500 x 1200 to 8000 x 4000 cells at heights 16 to 128).
+// 3000 x 1500 to 8000 x 4000 cells at heights 16 to 128).
 double fused_estimate(const Fused &fu, int fr);
 
 // Fused block heights the planner and the autotuner choose from.

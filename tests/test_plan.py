@@ -44,7 +44,8 @@ def test_partition_covers_each_cell_once(checker, fw, fr, nr, zs, zv):
 # grid (physical MX x MZ, TPML 20, stress imaging: MZ' = MZ + 2) -> the fused
 # block height bq_create takes, or 0 where the automatic choice leaves the grid
 # without fused tiles. Timed on MI355X (bench.py --mx X --mz Z with each height
-# forced): the model's choice is the fastest or within 1 % of it on each grid.
+# forced, and bq_autotune's timings): the model's choice is the fastest or within 4 %
+# of it on each grid.
 PICKS = {(400, 400): 0, (1500, 800): 0, (2000, 1000): 0, (2500, 1200): 0, (3000, 1400): 0,
          (3000, 1500): 24, (4000, 2000): 32, (5000, 2500): 48, (6000, 3000): 24, (8000, 4000): 128}
 
@@ -67,10 +68,10 @@ def _est(checker, nfx, rows, fr):
 
 def test_launch_estimate_closed_forms(checker):
     """n equal blocks of fr + 4 row steps: one wave of n <= 512 blocks takes
-    (fr + 4)(1.9 + 0.0039 n) us; 1024 take two full waves; a clipped last row
+    (fr + 4)(1.9 + 0.0035 n) us; 1024 take two full waves; a clipped last row
     of blocks is shorter."""
     for n in (1, 7, 512):
-        assert _est(checker, n, 64, 64) == pytest.approx(68 * (1.9 + 0.0039 * n), rel=1e-9)
-    assert _est(checker, 512, 128, 64) == pytest.approx(2 * 68 * (1.9 + 0.0039 * 512), rel=1e-9)
+        assert _est(checker, n, 64, 64) == pytest.approx(68 * (1.9 + 0.0035 * n), rel=1e-9)
+    assert _est(checker, 512, 128, 64) == pytest.approx(2 * 68 * (1.9 + 0.0035 * 512), rel=1e-9)
     # 2 blocks: 64 + 4 and 16 + 4 row steps; both run for 20, then one for 48
-    assert _est(checker, 1, 80, 64) == pytest.approx(20 * (1.9 + 0.0078) + 48 * (1.9 + 0.0039), rel=1e-9)
+    assert _est(checker, 1, 80, 64) == pytest.approx(20 * (1.9 + 0.0070) + 48 * (1.9 + 0.0035), rel=1e-9)
