@@ -125,9 +125,12 @@ __device__ __forceinline__ void fused2_block(const KParams &k, const Fused2 &fu,
   int fjm, fhjm = 0;
   auto fetch = [&](int s) {
     const int r = s * P;
-    pu = U[r + P + gx]; pw = Wf[r + 2 * P + gx]; pz = QZ[r + 2 * P + gx]; pq = QX[r + gx];
+    // own column: streaming hint, as in the single-step tiles
+    pu = stream_load(&U[r + P + gx]); pw = stream_load(&Wf[r + 2 * P + gx]);
+    pz = stream_load(&QZ[r + 2 * P + gx]); pq = stream_load(&QX[r + gx]);
     if (hl) { fhu = U[r + P + hlx]; fhw = Wf[r + 2 * P + hlx]; fhz = QZ[r + 2 * P + hlx]; fhq = QX[r + hlx]; }
-    ftxx = k.TXX0[r + gx]; ftzz = k.TZZ0[r + gx]; ftxz = k.TXZ0[r + gx]; fpr = k.PR0[r + gx];
+    ftxx = stream_load(&k.TXX0[r + gx]); ftzz = stream_load(&k.TZZ0[r + gx]);
+    ftxz = stream_load(&k.TXZ0[r + gx]); fpr = stream_load(&k.PR0[r + gx]);
     fjm = jm_at(k, r + gx);
     if (h1) {
       fhtxx = k.TXX0[r + h1x]; fhtzz = k.TZZ0[r + h1x]; fhtxz = k.TXZ0[r + h1x];
@@ -237,10 +240,10 @@ __device__ __forceinline__ void fused2_block(const KParams &k, const Fused2 &fu,
         bpp[0] = bpp[1]; bpp[1] = bpp[2]; bpp[2] = bpp[3]; bpp[3] = bpp[4]; bpp[4] = o[3];
         if (r2 >= a && r2 < e) {
           const int r = r2 * P + gx;
-          k.TXX1[r] = o[0];
-          k.TZZ1[r] = o[1];
-          k.TXZ1[r] = o[2];
-          k.PR1[r] = o[3];
+          stream_store(&k.TXX1[r], o[0]);
+          stream_store(&k.TZZ1[r], o[1]);
+          stream_store(&k.TXZ1[r], o[2]);
+          stream_store(&k.PR1[r], o[3]);
         }
       }
       if (h3) {
@@ -262,10 +265,10 @@ __device__ __forceinline__ void fused2_block(const KParams &k, const Fused2 &fu,
                     bxz[2], bxz[3], bzz[0], bzz[1], bzz[2], bzz[3], bpp[0], bpp[1], bpp[2], bpp[3],
                     L.vQ[l2 % 6][q], L.vU[l2 % 5][q], L.vZ[l2 % 6][q], L.vW[l2 % 6][q], jo[6], v);
       const int r = l2 * P + gx;
-      k.QX1[r] = v[0];
-      k.U1[r] = v[1];
-      k.QZ1[r] = v[2];
-      k.W1[r] = v[3];
+      stream_store(&k.QX1[r], v[0]);
+      stream_store(&k.U1[r], v[1]);
+      stream_store(&k.QZ1[r], v[2]);
+      stream_store(&k.W1[r], v[3]);
     }
   }
 }
