@@ -66,9 +66,11 @@ struct KParams {
 
 // Loads and stores with the streaming (non-temporal) hint, for the field rows
 // the fused tiles read in their own columns and the new rows they write (16
-// values per cell, about 2 GB per step on an 8040 x 4023 grid): they then do
-// not displace from L2 the material tables and the halo columns that
-// neighbouring tiles read at the same time. Measured on that grid, k_step:
+// values per cell, about 2 GB per step on an 8040 x 4023 grid), which no block
+// reads again in the launch. L2 hits and misses stay as they were; what
+// changes is below L2: the streamed rows no longer fill the last-level cache,
+// which then keeps the material tables and the halo rows and columns that L2
+// misses reach. Measured on that grid, k_step:
 // 519 us with plain loads and stores, 493 us with streaming stores, 463-466 us
 // with streaming stores and own-column loads; the hint on the halo-column and
 // material-index loads as well made it 478 us, on the own material index
